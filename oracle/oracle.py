@@ -1,0 +1,240 @@
+"""ctypes binding of the CPU oracle (libfk20_oracle.so).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, by the smoke check in __graft_entry__.py
+and by the cpu_baseline leg of bench.py, always as the checker or the timed CPU baseline.
+
+Arrays use the reference ABI (FK20-CUDA fr.cuh:24, g1.cuh:20-30) as numpy uint64:
+  fr  -> shape (..., 4)    g1p -> shape (..., 18)    g1a -> shape (..., 12)
+"""
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIBPATH = os.path.join(HERE, "libfk20_oracle.so")
+
+R_MOD = 0x73EDA753299D7D483339D80809A1D80553BDA402FFFE5BFEFFFFFFFF00000001
+P_MOD = 0x1A0111EA397FE69A4B1BA7B6434BACD764774B84F38512BF6730D2A0F6B0F6241EABFFFEB153FFFFB9FEFFFFFFFFAAAB
+
+
+def build():
+    subprocess.run(["make", "-s", "-C", HERE], check=True)
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIBPATH):
+            build()
+        _lib = ctypes.CDLL(LIBPATH)
+        _lib.or_default_threads.restype = ctypes.c_int
+        for fn in ["or_fr_eq", "or_fp_eq", "or_g1p_is_inf", "or_g1p_is_point", "or_g1p_eq"]:
+            getattr(_lib, fn).restype = ctypes.c_int
+    return _lib
+
+
+def _p(a):
+    assert a.dtype == np.uint64 and a.flags["C_CONTIGUOUS"]
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_uint64))
+
+
+# ---- integer <-> limb helpers ----
+
+def int_to_limbs(v, n):
+    return np.array([(v >> (64 * i)) & 0xFFFFFFFFFFFFFFFF for i in range(n)], dtype=np.uint64)
+
+
+def limbs_to_int(a):
+    v = 0
+    for i, w in enumerate(np.asarray(a, dtype=np.uint64).tolist()):
+        v |= int(w) << (64 * i)
+    return v
+
+
+def fr(v):
+    return int_to_limbs(v, 4)
+
+
+def fp(v):
+    return int_to_limbs(v, 6)
+
+
+def g1p_from_ints(x, y, z):
+    return np.concatenate([fp(x), fp(y), fp(z)])
+
+
+# ---- field ops ----
+
+def _binop(fn, n, x, y):
+    z = np.zeros(n, dtype=np.uint64)
+    getattr(lib(), fn)(_p(z), _p(np.ascontiguousarray(x, dtype=np.uint64)),
+                       _p(np.ascontiguousarray(y, dtype=np.uint64)))
+    return z
+
+
+def _unop(fn, n, x):
+    z = np.zeros(n, dtype=np.uint64)
+    getattr(lib(), fn)(_p(z), _p(np.ascontiguousarray(x, dtype=np.uint64)))
+    return z
+
+
+def fr_op(op, *args):
+    return (_binop if len(args) == 2 else _unop)("or_fr_" + op, 4, *args)
+
+
+def fp_op(op, *args):
+    return (_binop if len(args) == 2 else _unop)("or_fp_" + op, 6, *args)
+
+
+def fr_root_of_unity(n):
+    z = np.zeros(4, dtype=np.uint64)
+    lib().or_fr_root_of_unity(_p(z), ctypes.c_uint64(n))
+    return z
+
+
+def fr_roots515():
+    z = np.zeros((515, 4), dtype=np.uint64)
+    lib().or_fr_roots515(_p(z))
+    return z
+
+
+# ---- G1 ----
+
+def g1p_inf():
+    z = np.zeros(18, dtype=np.uint64)
+    lib().or_g1p_inf(_p(z))
+    return z
+
+
+def g1p_gen():
+    z = np.zeros(18, dtype=np.uint64)
+    lib().or_g1p_gen(_p(z))
+    return z
+
+
+def g1p_op(op, *args):
+    if op in ("add", "sub"):
+        return _binop("or_g1p_" + op, 18, *args)
+    if op == "add_affine":
+        return _binop("or_g1p_add_affine", 18, *args)
+    if op == "mul":
+        return _binop("or_g1p_mul", 18, *args)
+    return _unop("or_g1p_" + op, 18, *args)
+
+
+def g1p_eq(p, q):
+    return bool(lib().or_g1p_eq(_p(np.ascontiguousarray(p, dtype=np.uint64)),
+                                _p(np.ascontiguousarray(q, dtype=np.uint64))))
+
+
+def g1p_is_point(p):
+    return bool(lib().or_g1p_is_point(_p(np.ascontiguousarray(p, dtype=np.uint64))))
+
+
+def g1p_is_inf(p):
+    return bool(lib().or_g1p_is_inf(_p(np.ascontiguousarray(p, dtype=np.uint64))))
+
+
+def to_affine(p):
+    p = np.ascontiguousarray(p, dtype=np.uint64)
+    count = p.size // 18
+    a = np.zeros(p.shape[:-1] + (12,), dtype=np.uint64)
+    lib().or_g1p_to_affine_n(_p(a), _p(p), ctypes.c_int64(count))
+    return a
+
+
+# ---- transforms and FK20 ----
+
+def fr_fft(x, inverse=False):
+    x = np.ascontiguousarray(x, dtype=np.uint64)
+    n = x.shape[0]
+    out = np.zeros_like(x)
+    lib().or_fr_fft(_p(out), _p(x), ctypes.c_int(n), ctypes.c_int(int(inverse)))
+    return out
+
+
+def g1p_fft(x, inverse=False):
+    x = np.ascontiguousarray(x, dtype=np.uint64)
+    n = x.shape[0]
+    out = np.zeros_like(x)
+    lib().or_g1p_fft(_p(out), _p(x), ctypes.c_int(n), ctypes.c_int(int(inverse)))
+    return out
+
+
+def poly2toeplitz_coefficients(poly):
+    poly = np.ascontiguousarray(poly, dtype=np.uint64)
+    N = poly.shape[0]
+    tc = np.zeros((16, N // 8, 4), dtype=np.uint64)
+    lib().or_fk20_poly2toeplitz_coefficients(_p(tc), _p(poly), ctypes.c_int(N))
+    return tc
+
+
+def msm(tc_fft, xext_fft):
+    tc_fft = np.ascontiguousarray(tc_fft, dtype=np.uint64)
+    xext_fft = np.ascontiguousarray(xext_fft, dtype=np.uint64)
+    n2 = tc_fft.shape[1]
+    out = np.zeros((n2, 18), dtype=np.uint64)
+    lib().or_fk20_msm(_p(out), _p(tc_fft), _p(xext_fft), ctypes.c_int(n2))
+    return out
+
+
+def hext_fft2h_fft(hext_fft):
+    hext_fft = np.ascontiguousarray(hext_fft, dtype=np.uint64)
+    out = np.zeros_like(hext_fft)
+    lib().or_fk20_hext_fft2h_fft(_p(out), _p(hext_fft), ctypes.c_int(hext_fft.shape[0]))
+    return out
+
+
+def poly2h_fft(poly, xext_fft, nthreads=0):
+    """poly: (rows, N, 4) -> h_fft (rows, N/8, 18)."""
+    poly = np.ascontiguousarray(poly, dtype=np.uint64)
+    if poly.ndim == 2:
+        poly = poly[None]
+    rows, N = poly.shape[0], poly.shape[1]
+    out = np.zeros((rows, N // 8, 18), dtype=np.uint64)
+    lib().or_fk20_poly2h_fft(_p(out), _p(poly), _p(np.ascontiguousarray(xext_fft, dtype=np.uint64)),
+                             ctypes.c_int(N), ctypes.c_int(rows), ctypes.c_int(nthreads))
+    return out
+
+
+def setup_srs(tau, count):
+    out = np.zeros((count, 18), dtype=np.uint64)
+    lib().or_setup_srs(_p(out), _p(fr(tau)), ctypes.c_int(count))
+    return out
+
+
+def setup2xext_fft(srs, N):
+    out = np.zeros((16, N // 8, 18), dtype=np.uint64)
+    lib().or_setup2xext_fft(_p(out), _p(np.ascontiguousarray(srs, dtype=np.uint64)), ctypes.c_int(N))
+    return out
+
+
+def xext_fft_from_tau(tau, N):
+    out = np.zeros((16, N // 8, 18), dtype=np.uint64)
+    lib().or_xext_fft_from_tau(_p(out), _p(fr(tau)), ctypes.c_int(N))
+    return out
+
+
+def fk20_shadow(poly, tau):
+    poly = np.ascontiguousarray(poly, dtype=np.uint64)
+    N = poly.shape[0]
+    out = np.zeros((N // 8, 4), dtype=np.uint64)
+    lib().or_fk20_shadow(_p(out), _p(poly), _p(fr(tau)), ctypes.c_int(N))
+    return out
+
+
+def kzg_quotient_at(poly, c, tau):
+    poly = np.ascontiguousarray(poly, dtype=np.uint64)
+    out = np.zeros(4, dtype=np.uint64)
+    lib().or_kzg_quotient_at(_p(out), _p(poly), ctypes.c_int(poly.shape[0]),
+                             _p(np.ascontiguousarray(c, dtype=np.uint64)), _p(fr(tau)))
+    return out
+
+
+def default_threads():
+    return int(lib().or_default_threads())

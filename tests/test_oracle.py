@@ -1,0 +1,294 @@
+"""Pins the CPU oracle: reference KATs, closed forms (scalar shadow, KZG quotient openings),
+brute-force transforms and algebraic invariants.  CPU only.
+
+Each check is chosen so that a plausible mistake (dropped term, wrong sign or index,
+transposed operand) fails it; the `test_planted_*` cases plant such mistakes in the
+inputs of the oracle and confirm the checks catch them.
+"""
+import numpy as np
+import pytest
+
+import oracle as O
+from conftest import hex_fp, hex_fr, hex_g1p, load_golden
+from fk20_amd import synthetic as S
+
+R = O.R_MOD
+P = O.P_MOD
+
+
+def ints(a):
+    return [O.limbs_to_int(x) for x in a]
+
+
+# ---------------------------------------------------------------- Fp / Fr KATs
+
+FP_UNARY = {"fp_reduce6": "reduce", "fp_neg": "neg", "fp_sqr": "sqr", "fp_inv": "inv"}
+FR_UNARY = {"fr_reduce4": "reduce", "fr_neg": "neg", "fr_sqr": "sqr", "fr_inv": "inv"}
+
+
+@pytest.mark.parametrize("field", ["fp", "fr"])
+def test_field_kats(field):
+    """fptest_kat.cu:18-716 / frtest_kat.cu:15-650, compared modulo the field prime."""
+    kat = load_golden(field + "_kat.json")["ops"]
+    mod = P if field == "fp" else R
+    conv = hex_fp if field == "fp" else hex_fr
+    op = O.fp_op if field == "fp" else O.fr_op
+    unary = FP_UNARY if field == "fp" else FR_UNARY
+    checked = 0
+    for name, arrs in kat.items():
+        if name in unary:
+            for q, a in zip(arrs["q"], arrs["a"]):
+                got = O.limbs_to_int(op(unary[name], conv(q)))
+                assert got == int(a, 16) % mod, name
+                checked += 1
+        elif name.endswith(("_x2", "_x3")):
+            m = 2 if name.endswith("_x2") else 3
+            for q, a in zip(arrs["q"], arrs["a"]):
+                got = O.limbs_to_int(op("mul", conv(q), conv(hex(m))))
+                assert got == int(a, 16) % mod, name
+                checked += 1
+        elif name.endswith(("_add", "_sub", "_mul")):
+            fn = name.split("_")[1]
+            for x, y, a in zip(arrs["x"], arrs["y"], arrs["a"]):
+                got = O.limbs_to_int(op(fn, conv(x), conv(y)))
+                assert got == int(a, 16) % mod, name
+                checked += 1
+        elif name.endswith("_eq"):
+            for i, t in enumerate(arrs["t"]):
+                for j, u in enumerate(arrs["u"]):
+                    lhs = O.limbs_to_int(op("reduce", conv(t)))
+                    rhs = O.limbs_to_int(op("reduce", conv(u)))
+                    assert (lhs == rhs) == (i == j)
+                    checked += 1
+        elif name == "mandelbrot":
+            # z <- z^2 + x, 10000 times, from z = 0 (fptest_kat.cu:666-700)
+            for q, a in zip(arrs["q"], arrs["a"]):
+                x = int(q, 16) % P
+                z = 0
+                for _ in range(10000):
+                    z = (z * z + x) % P
+                assert z == int(a, 16) % P
+                checked += 1
+    assert checked > 60
+
+
+def test_field_mul_matches_python_ints():
+    rng = np.random.default_rng(7)
+    for _ in range(200):
+        x = int.from_bytes(rng.bytes(48), "little")
+        y = int.from_bytes(rng.bytes(48), "little")
+        assert O.limbs_to_int(O.fp_op("mul", O.fp(x), O.fp(y))) == x * y % P
+        a, b = x >> 128, y >> 128
+        assert O.limbs_to_int(O.fr_op("mul", O.fr(a), O.fr(b))) == a * b % R
+        assert O.limbs_to_int(O.fr_op("inv", O.fr(a))) == pow(a, R - 2, R)
+
+
+# ---------------------------------------------------------------- G1 KATs
+
+def test_g1_kats():
+    """g1test_kat.cu:71-330: 0, 2G, 3G, 24G, 25G, wG, w^8 G."""
+    k = load_golden("g1_kat.json")
+    pt = lambda name: hex_g1p(k[name])
+    G = O.g1p_gen()
+    x0, x2, x3, x24, x25 = (pt(n) for n in ["g1p_x0", "g1p_x2", "g1p_x3", "g1p_x24", "g1p_x25"])
+    assert O.g1p_is_inf(x0) and O.g1p_is_point(x0)
+    assert O.g1p_eq(O.g1p_op("add", x0, x0), O.g1p_inf())
+    assert O.g1p_eq(O.g1p_op("add", G, x0), G) and O.g1p_eq(O.g1p_op("add", x0, G), G)
+    assert O.g1p_eq(O.g1p_op("add", G, G), x2) and O.g1p_eq(O.g1p_op("dbl", G), x2)
+    assert O.g1p_eq(O.g1p_op("add", G, x2), x3)
+    p = x3
+    for _ in range(3):
+        p = O.g1p_op("dbl", p)
+    assert O.g1p_eq(p, x24)
+    assert O.g1p_eq(O.g1p_op("add", O.g1p_op("sub", p, x2), x3), x25)
+    assert O.g1p_eq(O.g1p_op("mul", G, O.fr(25)), x25)
+    w = O.fr_root_of_unity(512)
+    assert O.g1p_eq(O.g1p_op("mul", G, w), pt("g1p_wG"))
+    assert O.g1p_eq(O.g1p_op("mul", G, O.fr_op("pow", w, O.fr(8))), pt("g1p_w8G"))
+    # mixed addition with an affine operand (g1test_kat.cu:300-330)
+    aff = O.to_affine(G)
+    assert O.g1p_eq(O.g1p_op("add_affine", G, aff), x2)
+    for q in [G, x2, x3, x24, x25]:
+        assert O.g1p_is_point(q)
+    # the raw 256-bit scalar: (k + r) P == k P for P in G1 (g1p_mul.cu:18-66)
+    assert O.g1p_eq(O.g1p_op("mul", G, O.fr(25 + R)), x25)
+    # r P = infinity, (r - 1) P = -P
+    assert O.g1p_is_inf(O.g1p_op("mul", G, O.fr(R)))
+    assert O.g1p_eq(O.g1p_op("mul", G, O.fr(R - 1)), O.g1p_op("neg", G))
+
+
+def test_g1_addsub_chain():
+    """g1test_kat.cu:262-290: p,q <- addsub twice equals dbl chains."""
+    G = O.g1p_gen()
+    p, q, r = G, G, G
+    for _ in range(20):
+        p, q = O.g1p_op("add", p, q), O.g1p_op("sub", p, q)
+        p, q = O.g1p_op("add", p, q), O.g1p_op("sub", p, q)
+        r = O.g1p_op("dbl", r)
+        assert O.g1p_eq(p, q) and O.g1p_eq(q, r)
+
+
+# ---------------------------------------------------------------- roots and transforms
+
+def test_roots_table():
+    """fr_roots.cu:17-533; entries 513/514 = 2^-9, 2^-9 w^-128."""
+    ref = [int(h, 16) for h in load_golden("fr_roots.json")["roots"]]
+    assert ints(O.fr_roots515()) == ref
+    w = ref[1]
+    assert pow(w, 512, R) == 1 and pow(w, 256, R) == R - 1
+    assert ref[513] * 512 % R == 1 and ref[514] * 512 * pow(w, 128, R) % R == 1
+
+
+def brute_dft(vals, n, inverse=False):
+    w = pow(5, (R - 1) // n, R)
+    if inverse:
+        w = pow(w, R - 2, R)
+    out = [sum(vals[j] * pow(w, i * j, R) for j in range(n)) % R for i in range(n)]
+    if inverse:
+        ninv = pow(n, R - 2, R)
+        out = [x * ninv % R for x in out]
+    return out
+
+
+def test_fr_fft_kat():
+    """fr_fft_testvector.cu: a = fft(q) (checked by frtest_fft.cu:35-86)."""
+    k = load_golden("fr_fft_kat.json")
+    q = np.array([hex_fr(h) for h in k["q"]])
+    a = np.array([hex_fr(h) for h in k["a"]])
+    assert ints(O.fr_fft(q)) == [int(h, 16) % R for h in k["a"]]
+    assert ints(O.fr_fft(a, inverse=True)) == [int(h, 16) % R for h in k["q"]]
+
+
+@pytest.mark.parametrize("n", [4, 8, 128])
+def test_fr_fft_brute(n):
+    vals = S.random_fr(n, 100 + n)
+    v = ints(vals)
+    assert ints(O.fr_fft(vals)) == brute_dft(v, n)
+    assert ints(O.fr_fft(vals, inverse=True)) == brute_dft(v, n, inverse=True)
+
+
+def test_g1_fft_kat():
+    """fk20test_kat.cu: g1p_correct = g1p_fft(g1p_input), and the inverse."""
+    k = load_golden("g1p_fft_kat.json")
+    gin = np.array([hex_g1p(p) for p in k["input"]])
+    gout = np.array([hex_g1p(p) for p in k["output"]])
+    out = O.g1p_fft(gin)
+    assert all(O.g1p_eq(out[i], gout[i]) for i in range(512))
+    back = O.g1p_fft(gout, inverse=True)
+    assert all(O.g1p_eq(back[i], gin[i]) for i in range(512))
+
+
+def test_g1_fft_commutes_with_scalars():
+    """FFT([x]G) = [FFT(x)]G (g1test_fft.cu:137-366 invariants), n = 16."""
+    n = 16
+    xs = S.random_fr(n, 5)
+    G = O.g1p_gen()
+    pts = np.array([O.g1p_op("mul", G, x) for x in xs])
+    out = O.g1p_fft(pts)
+    ref = O.fr_fft(xs)
+    assert all(O.g1p_eq(out[i], O.g1p_op("mul", G, ref[i])) for i in range(n))
+    back = O.g1p_fft(out, inverse=True)
+    assert all(O.g1p_eq(back[i], pts[i]) for i in range(n))
+
+
+# ---------------------------------------------------------------- FK20 closed forms
+
+@pytest.fixture(scope="module")
+def small_case():
+    N = 1024
+    tau = S.random_secret(11)
+    poly = S.random_polys(1, N, 12)[0]
+    xext = O.xext_fft_from_tau(tau, N)
+    return N, tau, poly, xext
+
+
+def test_toeplitz_layout(small_case):
+    """FK20Py fk20_multi.py:92-93 layout, checked index by index."""
+    N, _, poly, _ = small_case
+    tc = O.poly2toeplitz_coefficients(poly)
+    k = N // 16
+    p = ints(poly)
+    for i in range(16):
+        row = ints(tc[i])
+        exp = [p[N - 1 - i]] + [0] * (k + 1) + [p[j] for j in range(2 * 16 - i - 1, N - 16 - i, 16)]
+        assert row == exp
+
+
+def test_setup_paths_agree():
+    """xext_fft from the SRS (FK20Py toeplitz_part1) equals the scalar-shadow [xhat]G."""
+    N = 1024
+    tau = S.random_secret(3)
+    srs = O.setup_srs(tau, N + 1)
+    a = O.setup2xext_fft(srs, N).reshape(-1, 18)
+    b = O.xext_fft_from_tau(tau, N).reshape(-1, 18)
+    assert all(O.g1p_eq(x, y) for x, y in zip(a, b))
+
+
+def test_fk20_row_matches_shadow_and_kzg(small_case):
+    N, tau, poly, xext = small_case
+    h = O.poly2h_fft(poly, xext, nthreads=0)[0]
+    s = O.fk20_shadow(poly, tau)
+    G = O.g1p_gen()
+    assert all(O.g1p_eq(h[k], O.g1p_op("mul", G, s[k])) for k in range(N // 8))
+    # KZG: s[k] = q_k(tau), q_k = f div (X^16 - w_{N/8}^k) (kzg_proofs.py:198-205)
+    w = O.limbs_to_int(O.fr_root_of_unity(N // 8))
+    for k in range(0, N // 8, 9):
+        q = O.kzg_quotient_at(poly, O.fr(pow(w, k, R)), tau)
+        assert O.limbs_to_int(q) == O.limbs_to_int(s[k])
+
+
+def test_shadow_equals_kzg_all_sizes():
+    for N in (1024, 2048, 4096):
+        tau = S.random_secret(N)
+        poly = S.random_polys(1, N, N + 1)[0]
+        s = O.fk20_shadow(poly, tau)
+        w = O.limbs_to_int(O.fr_root_of_unity(N // 8))
+        for k in (0, 1, N // 16, N // 8 - 1):
+            q = O.kzg_quotient_at(poly, O.fr(pow(w, k, R)), tau)
+            assert O.limbs_to_int(q) == O.limbs_to_int(s[k]), (N, k)
+
+
+def test_stage_split_equals_pipeline(small_case):
+    N, _, poly, xext = small_case
+    tc = O.poly2toeplitz_coefficients(poly)
+    tc_fft = np.array([O.fr_fft(tc[i]) for i in range(16)])
+    hext = O.msm(tc_fft, xext)
+    h_fft = O.hext_fft2h_fft(hext)
+    ref = O.poly2h_fft(poly, xext)[0]
+    assert all(O.g1p_eq(a, b) for a, b in zip(h_fft, ref))
+
+
+def test_zero_polynomial_gives_infinity(small_case):
+    N, _, _, xext = small_case
+    h = O.poly2h_fft(np.zeros((N, 4), dtype=np.uint64), xext)[0]
+    assert all(O.g1p_is_inf(p) for p in h)
+
+
+# ---------------------------------------------------------------- planted mistakes
+
+def test_planted_coefficient_change_is_detected(small_case):
+    """A changed coefficient (a dropped or wrong term) breaks the KZG closed form."""
+    N, tau, poly, _ = small_case
+    s = O.fk20_shadow(poly, tau)
+    bad = poly.copy()
+    bad[N - 20] = O.fr((O.limbs_to_int(bad[N - 20]) + 1) % R)
+    q = O.kzg_quotient_at(bad, O.fr(1), tau)
+    assert O.limbs_to_int(q) != O.limbs_to_int(s[0])
+
+
+def test_planted_index_shift_is_detected(small_case):
+    """Opening at the wrong coset (index off by one) does not match."""
+    N, tau, poly, _ = small_case
+    s = O.fk20_shadow(poly, tau)
+    w = O.limbs_to_int(O.fr_root_of_unity(N // 8))
+    q = O.kzg_quotient_at(poly, O.fr(pow(w, 3, R)), tau)
+    assert O.limbs_to_int(q) != O.limbs_to_int(s[2])
+
+
+def test_planted_sign_flip_is_detected():
+    """The inverse transform with the wrong sign of the root does not invert."""
+    n = 8
+    vals = S.random_fr(n, 9)
+    fwd = O.fr_fft(vals)
+    assert ints(O.fr_fft(fwd, inverse=True)) == ints(vals)
+    assert ints(O.fr_fft(O.fr_fft(fwd))) != ints(vals)
