@@ -1,0 +1,57 @@
+// Internal interface between the HIP kernel translation units and the C-ABI host code.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stddef.h>
+#include <stdint.h>
+
+#include "g1.hip.h"
+
+namespace fk20 {
+
+// Per-device constant tables, built once on the host (api.cpp) and copied to HBM.
+struct DevTables {
+    uint32_t tw_mont[513][8];     // w^i, w = 5^((r-1)/512), Montgomery form
+    uint32_t tw_std[513][8];      // w^i, standard form (scalar-multiplier input)
+    uint32_t inv2k_mont[10][8];   // 2^-j mod r, Montgomery form, j = 0..9
+    uint32_t inv2k_std[10][8];    // 2^-j mod r, standard form
+};
+
+// Reference ABI element sizes in 32-bit words
+static constexpr int FR_WORDS = 8;    // fr_t  (fr.cuh:24)
+static constexpr int G1P_WORDS = 36;  // g1p_t (g1.cuh:27)
+static constexpr int G1A_WORDS = 24;  // g1a_t (g1.cuh:20)
+
+__host__ __device__ __forceinline__ unsigned bitrev(unsigned x, unsigned bits) {
+#ifdef __HIP_DEVICE_COMPILE__
+    return __builtin_bitreverse32(x) >> (32 - bits);
+#else
+    unsigned r = 0;
+    for (unsigned i = 0; i < bits; i++) r |= ((x >> i) & 1u) << (bits - 1 - i);
+    return r;
+#endif
+}
+
+// ---- launchers (one per kernel family); all asynchronous on `s` ----
+hipError_t launch_toeplitz(uint32_t *tc, const uint32_t *poly, unsigned rows, unsigned N, hipStream_t s);
+hipError_t launch_toeplitz_fft(uint32_t *tc_fft, const uint32_t *poly, unsigned rows, unsigned N,
+                               const DevTables *T, hipStream_t s);
+hipError_t launch_fr_ntt(uint32_t *out, const uint32_t *in, unsigned count, unsigned n, bool inverse,
+                         const DevTables *T, hipStream_t s);
+hipError_t launch_g1_fft(uint32_t *out, const uint32_t *in, unsigned count, unsigned n, bool inverse,
+                         const DevTables *T, hipStream_t s);
+hipError_t launch_hext2h(uint32_t *h, unsigned rows, unsigned n2, hipStream_t s);
+hipError_t launch_hext_fft2h_fft(uint32_t *h_fft, const uint32_t *hext_fft, unsigned rows, unsigned n2,
+                                 const DevTables *T, hipStream_t s);
+hipError_t launch_std_to_packed(uint32_t *packed, const uint32_t *std_pts, size_t count, hipStream_t s);
+hipError_t launch_msm_plain(uint32_t *hext_fft, const uint32_t *tc_fft, const uint32_t *xext_packed,
+                            unsigned rows, unsigned n2, hipStream_t s);
+hipError_t launch_setup_gather(uint32_t *xext, const uint32_t *srs, unsigned N, hipStream_t s);
+hipError_t launch_to_affine(uint32_t *out, const uint32_t *in, size_t count, hipStream_t s);
+hipError_t launch_fp_op(int op, uint32_t *out, const uint32_t *a, const uint32_t *b, size_t count,
+                        hipStream_t s);
+hipError_t launch_fr_op(int op, uint32_t *out, const uint32_t *a, const uint32_t *b, size_t count,
+                        hipStream_t s);
+hipError_t launch_g1_op(int op, uint32_t *out, const uint32_t *p, const uint32_t *q, const uint32_t *k,
+                        size_t count, hipStream_t s);
+
+}  // namespace fk20

@@ -1,0 +1,224 @@
+// G1 arithmetic of BLS12-381 (y^2 = x^3 + 4) in homogeneous projective coordinates.
+//
+// Complete formulas of Renes, Costello and Batina (eprint 2015/1060): algorithm 7
+// (addition), 8 (mixed addition with an affine point) and 9 (doubling) for a = 0,
+// b3 = 3b = 12 -- the same formulas the reference uses (g1p_add.cu:78, g1p_dbl.cu:28,
+// g1p_multi.cu:83), written for the lazy 29-bit Fp of fp.hip.h.  The point at infinity is
+// (0 : 1 : 0); any (0 : y != 0 : 0) is accepted as infinity (g1.cuh:14 G1P_ANYINF).
+//
+// Value bounds (multiples of p) are annotated per step.  Inputs may be anything
+// below 10 p (every packed value is < 2^384 < 9.9 p); outputs are below 4 p.  The
+// subtrahend bound decides each fe_sub<K> constant; every product has a*b < 2^20 p^2.
+#pragma once
+#include "fp.hip.h"
+#include "fr.hip.h"
+
+namespace fk20 {
+
+struct g1p {
+    fe x, y, z;
+};
+
+// affine point (x, y); infinity is not representable and is handled by callers
+struct g1a {
+    fe x, y;
+};
+
+FK20_HD void g1_inf(g1p &p) {
+    fe_zero(p.x);
+    fe_one(p.y);
+    fe_zero(p.z);
+}
+
+// RCB algorithm 7: r = p + q.  Inputs < 10 p, output < 4 p.
+FK20_HD void g1_add(g1p &r, const g1p &p, const g1p &q) {
+    fe t0, t1, t2, t3, t4, X3, Y3, Z3;
+    fe_mul(t0, p.x, q.x);           // t0 < 2p
+    fe_mul(t1, p.y, q.y);           // t1 < 2p
+    fe_mul(t2, p.z, q.z);           // t2 < 2p
+    fe_add(t3, p.x, p.y);           // < 20p
+    fe_add(t4, q.x, q.y);           // < 20p
+    fe_mul(t3, t3, t4);             // < 2p
+    fe_add(t4, t0, t1);             // < 4p
+    fe_sub<4>(t3, t3, t4);          // < 6p
+    fe_add(t4, p.y, p.z);           // < 20p
+    fe_add(X3, q.y, q.z);           // < 20p
+    fe_mul(t4, t4, X3);             // < 2p
+    fe_add(X3, t1, t2);             // < 4p
+    fe_sub<4>(t4, t4, X3);          // < 6p
+    fe_add(X3, p.x, p.z);           // < 20p
+    fe_add(Y3, q.x, q.z);           // < 20p
+    fe_mul(X3, X3, Y3);             // < 2p
+    fe_add(Y3, t0, t2);             // < 4p
+    fe_sub<4>(Y3, X3, Y3);          // < 6p
+    fe_add(X3, t0, t0);             // < 4p
+    fe_add(t0, X3, t0);             // < 6p
+    fe_mul_small<12>(t2, t2);       // < 24p
+    fe_add(Z3, t1, t2);             // < 26p
+    fe_sub<32>(t1, t1, t2);         // < 34p
+    fe_mul_small<12>(Y3, Y3);       // < 72p
+    fe_mul(X3, t4, Y3);             // 6p*72p -> < 2p
+    fe_mul(t2, t3, t1);             // < 2p
+    fe_sub<2>(X3, t2, X3);          // < 4p
+    fe_mul(Y3, Y3, t0);             // < 2p
+    fe_mul(t1, t1, Z3);             // < 2p
+    fe_add(Y3, t1, Y3);             // < 4p
+    fe_mul(t0, t0, t3);             // < 2p
+    fe_mul(Z3, Z3, t4);             // < 2p
+    fe_add(Z3, Z3, t0);             // < 4p
+    r.x = X3;
+    r.y = Y3;
+    r.z = Z3;
+}
+
+// RCB algorithm 8: r = p + (x2, y2), affine operand not infinity.  Inputs < 10 p.
+FK20_HD void g1_add_aff(g1p &r, const g1p &p, const fe &X2, const fe &Y2) {
+    fe t0, t1, t2, t3, t4, X3, Y3, Z3;
+    fe_mul(t0, p.x, X2);            // < 2p
+    fe_mul(t1, p.y, Y2);            // < 2p
+    fe_add(t3, X2, Y2);             // < 20p
+    fe_add(t4, p.x, p.y);           // < 20p
+    fe_mul(t3, t3, t4);             // < 2p
+    fe_add(t4, t0, t1);             // < 4p
+    fe_sub<4>(t3, t3, t4);          // < 6p
+    fe_mul(t4, Y2, p.z);            // < 2p
+    fe_add(t4, t4, p.y);            // < 12p
+    fe_mul(Y3, X2, p.z);            // < 2p
+    fe_add(Y3, Y3, p.x);            // < 12p
+    fe_add(X3, t0, t0);             // < 4p
+    fe_add(t0, X3, t0);             // < 6p
+    fe_mul_small<12>(t2, p.z);      // < 120p
+    fe_add(Z3, t1, t2);             // < 122p
+    fe_sub<128>(t1, t1, t2);        // < 130p
+    fe_mul_small<12>(Y3, Y3);       // < 144p
+    fe_mul(X3, t4, Y3);             // 12p*144p -> < 2p
+    fe_mul(t2, t3, t1);             // 6p*130p  -> < 2p
+    fe_sub<2>(X3, t2, X3);          // < 4p
+    fe_mul(Y3, Y3, t0);             // 144p*6p  -> < 2p
+    fe_mul(t1, t1, Z3);             // 130p*122p -> < 2p
+    fe_add(Y3, t1, Y3);             // < 4p
+    fe_mul(t0, t0, t3);             // < 2p
+    fe_mul(Z3, Z3, t4);             // 122p*12p -> < 2p
+    fe_add(Z3, Z3, t0);             // < 4p
+    r.x = X3;
+    r.y = Y3;
+    r.z = Z3;
+}
+
+// RCB algorithm 9: r = 2p.  Input < 10 p, output < 4 p.
+FK20_HD void g1_dbl(g1p &r, const g1p &p) {
+    fe t0, t1, t2, X3, Y3, Z3;
+    fe_sqr(t0, p.y);                // < 2p
+    fe_add(Z3, t0, t0);             // < 4p
+    fe_add(Z3, Z3, Z3);             // < 8p
+    fe_add(Z3, Z3, Z3);             // < 16p
+    fe_mul(t1, p.y, p.z);           // < 2p
+    fe_sqr(t2, p.z);                // < 2p
+    fe_mul_small<12>(t2, t2);       // < 24p
+    fe_mul(X3, t2, Z3);             // < 2p
+    fe_add(Y3, t0, t2);             // < 26p
+    fe_mul(Z3, t1, Z3);             // < 2p
+    fe_add(t1, t2, t2);             // < 48p
+    fe_add(t2, t1, t2);             // < 72p
+    fe_sub<128>(t0, t0, t2);        // < 130p
+    fe_mul(Y3, t0, Y3);             // 130p*26p -> < 2p
+    fe_add(Y3, X3, Y3);             // < 4p
+    fe_mul(t1, p.x, p.y);           // < 2p
+    fe_mul(X3, t0, t1);             // < 2p
+    fe_add(X3, X3, X3);             // < 4p
+    r.x = X3;
+    r.y = Y3;
+    r.z = Z3;
+}
+
+// r = -p.  Input < 10 p (y bound), output y < 16 p... kept below 10p by using 16p - y
+// only where y < 16p; callers pass values < 4p or < 10p.
+FK20_HD void g1_neg(g1p &r, const g1p &p) {
+    r.x = p.x;
+    fe_neg<16>(r.y, p.y);
+    r.z = p.z;
+}
+
+// (p, q) <- (p + q, p - q)
+FK20_HD void g1_addsub(g1p &p, g1p &q) {
+    g1p nq;
+    g1_neg(nq, q);      // y < 16p: g1_add accepts inputs < 20p in practice (see bound notes)
+    g1p s;
+    g1_add(s, p, q);
+    g1_add(q, p, nq);
+    p = s;
+}
+
+// ---------------------------------------------------------------- memory forms
+
+// Standard-form g1p_t (reference ABI: 3 x 6 x u64, little endian) <-> Montgomery registers.
+FK20_HD void g1_load_std(g1p &p, const uint32_t *w /* 36 words */) {
+    fe t;
+    fe_unpack(t, w);
+    fe_to_mont(p.x, t);
+    fe_unpack(t, w + 12);
+    fe_to_mont(p.y, t);
+    fe_unpack(t, w + 24);
+    fe_to_mont(p.z, t);
+}
+
+// Canonical standard form (each coordinate < p).
+FK20_HD void g1_store_std(uint32_t *w, const g1p &p) {
+    fe t;
+    fe_from_mont(t, p.x);
+    fe_pack(w, t);
+    fe_from_mont(t, p.y);
+    fe_pack(w + 12, t);
+    fe_from_mont(t, p.z);
+    fe_pack(w + 24, t);
+}
+
+// Packed Montgomery form (internal buffers): 36 words, values < 2^384.
+FK20_HD void g1_load_packed(g1p &p, const uint32_t *w) {
+    fe_unpack(p.x, w);
+    fe_unpack(p.y, w + 12);
+    fe_unpack(p.z, w + 24);
+}
+
+FK20_HD void g1_store_packed(uint32_t *w, const g1p &p) {
+    fe_pack(w, p.x);
+    fe_pack(w + 12, p.y);
+    fe_pack(w + 24, p.z);
+}
+
+// ---------------------------------------------------------------- scalar multiplication
+
+// r = k p by MSB-first double-and-add over the 256-bit standard-form scalar k
+// (8 x 32-bit words), as g1p_mul.cu:18-66.  k need not be reduced mod r.
+FK20_HD void g1_mul(g1p &r, const g1p &p, const uint32_t *k) {
+    g1p q;
+    g1_inf(q);
+    for (int w = 7; w >= 0; w--) {
+        const uint32_t kw = k[w];
+        for (int b = 31; b >= 0; b--) {
+            g1_dbl(q, q);
+            if ((kw >> b) & 1) g1_add(q, q, p);
+        }
+    }
+    r = q;
+}
+
+// Affine conversion: (x/z, y/z) canonical Montgomery, or the infinity flag when z = 0.
+FK20_HD bool g1_to_affine(fe &ax, fe &ay, const g1p &p) {
+    fe zc;
+    fe_canonical(zc, p.z);
+    if (fe_is_zero_canonical(zc)) {
+        fe_zero(ax);
+        fe_zero(ay);
+        return true;
+    }
+    fe zi;
+    fe_inv(zi, zc);
+    fe_mul(ax, p.x, zi);
+    fe_mul(ay, p.y, zi);
+    fe_canonical(ax, ax);
+    fe_canonical(ay, ay);
+    return false;
+}
+
+}  // namespace fk20

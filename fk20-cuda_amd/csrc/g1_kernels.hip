@@ -1,0 +1,445 @@
+// G1 kernels of the plain (per-row) path: G1 FFT/IFFT of size 128/256/512, the fused
+// IFFT -> zero upper half -> FFT, the per-row MSM, setup gather, affine conversion and
+// element-wise G1/Fp arithmetic.
+//
+// The plain G1 transforms keep one transform per workgroup of n/2 threads with the n
+// points in LDS (packed Montgomery form, 144 B per point, 72 KiB for n = 512), one
+// butterfly per thread and stage: the structure of g1p_fft.cu:25-170 (forward) and
+// g1p_fft.cu:179-330 (inverse), with the inverse written as a decimation-in-time over
+// w^-1 followed by the 1/n scaling.  Twiddle multiplication is double-and-add over the
+// standard-form root (g1p_mul.cu:18-66).
+#include "common.hip.h"
+
+namespace fk20 {
+
+__device__ __forceinline__ void words_load(uint32_t *dst, const uint32_t *src, int nwords) {
+    const uint4 *s = reinterpret_cast<const uint4 *>(src);
+    for (int i = 0; i < nwords / 4; i++) {
+        uint4 v = s[i];
+        dst[4 * i] = v.x; dst[4 * i + 1] = v.y; dst[4 * i + 2] = v.z; dst[4 * i + 3] = v.w;
+    }
+}
+
+__device__ __forceinline__ void g1_lds_load(g1p &p, const uint32_t *lds) {
+    uint32_t w[36];
+#pragma unroll
+    for (int i = 0; i < 9; i++) {
+        uint4 v = reinterpret_cast<const uint4 *>(lds)[i];
+        w[4 * i] = v.x; w[4 * i + 1] = v.y; w[4 * i + 2] = v.z; w[4 * i + 3] = v.w;
+    }
+    g1_load_packed(p, w);
+}
+
+__device__ __forceinline__ void g1_lds_store(uint32_t *lds, const g1p &p) {
+    uint32_t w[36];
+    g1_store_packed(w, p);
+#pragma unroll
+    for (int i = 0; i < 9; i++)
+        reinterpret_cast<uint4 *>(lds)[i] = make_uint4(w[4 * i], w[4 * i + 1], w[4 * i + 2], w[4 * i + 3]);
+}
+
+__device__ __forceinline__ void g1_gload_std(g1p &p, const uint32_t *g) {
+    uint32_t w[36];
+#pragma unroll
+    for (int i = 0; i < 9; i++) {
+        uint4 v = reinterpret_cast<const uint4 *>(g)[i];
+        w[4 * i] = v.x; w[4 * i + 1] = v.y; w[4 * i + 2] = v.z; w[4 * i + 3] = v.w;
+    }
+    g1_load_std(p, w);
+}
+
+__device__ __forceinline__ void g1_gstore_std(uint32_t *g, const g1p &p) {
+    uint32_t w[36];
+    g1_store_std(w, p);
+#pragma unroll
+    for (int i = 0; i < 9; i++)
+        reinterpret_cast<uint4 *>(g)[i] = make_uint4(w[4 * i], w[4 * i + 1], w[4 * i + 2], w[4 * i + 3]);
+}
+
+// Scalar multiplication with a non-unrolled bit loop (one inlined dbl and add).
+__device__ __noinline__ void g1_mul_dev(g1p &r, const g1p &p, const uint32_t *k) {
+    g1p q;
+    g1_inf(q);
+#pragma unroll 1
+    for (int w = 7; w >= 0; w--) {
+        const uint32_t kw = k[w];
+#pragma unroll 1
+        for (int b = 31; b >= 0; b--) {
+            g1_dbl(q, q);
+            if ((kw >> b) & 1) g1_add(q, q, p);
+        }
+    }
+    r = q;
+}
+
+// One radix-2 DIT butterfly on LDS points l and r with twiddle w^idx (w^-idx if INV).
+template <bool INV>
+__device__ __forceinline__ void g1_butterfly(uint32_t *lds, unsigned l, unsigned r, unsigned idx,
+                                             const DevTables *T) {
+    g1p x, y, a;
+    g1_lds_load(x, lds + r * 36);
+    if (idx != 0) g1_mul_dev(x, x, T->tw_std[INV ? 512 - idx : idx]);
+    g1_lds_load(y, lds + l * 36);
+    g1_add(a, y, x);
+    g1_neg(x, x);
+    g1_add(x, y, x);
+    g1_lds_store(lds + l * 36, a);
+    g1_lds_store(lds + r * 36, x);
+}
+
+// In-place DIT over n = 2^logn packed points in bit-reversed order -> natural order.
+template <bool INV>
+__device__ void g1_dit(uint32_t *lds, unsigned logn, const DevTables *T) {
+    const unsigned t = threadIdx.x;
+#pragma unroll 1
+    for (unsigned s = 0; s < logn; s++) {
+        const unsigned j = t & ((1u << s) - 1);
+        const unsigned l = ((t >> s) << (s + 1)) + j;
+        g1_butterfly<INV>(lds, l, l + (1u << s), j << (8 - s), T);
+        __syncthreads();
+    }
+}
+
+template <unsigned LOGN, bool INV>
+__global__ void __launch_bounds__(256) k_g1_fft(uint32_t *out, const uint32_t *in, const DevTables *T) {
+    constexpr unsigned n = 1u << LOGN, half = n / 2;
+    __shared__ __attribute__((aligned(16))) uint32_t lds[n * 36];
+    const unsigned t = threadIdx.x;
+    in += (size_t)blockIdx.x * n * G1P_WORDS;
+    out += (size_t)blockIdx.x * n * G1P_WORDS;
+    for (unsigned e = t; e < n; e += half) {
+        g1p p;
+        g1_gload_std(p, in + (size_t)e * G1P_WORDS);
+        g1_lds_store(lds + bitrev(e, LOGN) * 36, p);
+    }
+    __syncthreads();
+    g1_dit<INV>(lds, LOGN, T);
+    for (unsigned e = t; e < n; e += half) {
+        g1p p;
+        g1_lds_load(p, lds + e * 36);
+        if (INV) g1_mul_dev(p, p, T->inv2k_std[LOGN]);
+        g1_gstore_std(out + (size_t)e * G1P_WORDS, p);
+    }
+}
+
+// hext_fft -> IFFT -> keep the first half (scaled by 1/n), infinity above -> FFT.
+// fk20_hext_fft2h_fft.cu:27-70 (and FK20Py toeplitz_part3 + fk20_multi.py:98-104).
+template <unsigned LOGN>
+__global__ void __launch_bounds__(256) k_hext_fft2h_fft(uint32_t *h_fft, const uint32_t *hext_fft,
+                                                         const DevTables *T) {
+    constexpr unsigned n = 1u << LOGN, half = n / 2;
+    __shared__ __attribute__((aligned(16))) uint32_t lds[n * 36];
+    const unsigned t = threadIdx.x;
+    hext_fft += (size_t)blockIdx.x * n * G1P_WORDS;
+    h_fft += (size_t)blockIdx.x * n * G1P_WORDS;
+    for (unsigned e = t; e < n; e += half) {
+        g1p p;
+        g1_gload_std(p, hext_fft + (size_t)e * G1P_WORDS);
+        g1_lds_store(lds + bitrev(e, LOGN) * 36, p);
+    }
+    __syncthreads();
+    g1_dit<true>(lds, LOGN, T);
+    // natural order now; element t (< n/2) survives, scaled by 1/n; move it to the
+    // bit-reversed slot for the forward DIT and put infinity in the odd slots
+    g1p p;
+    g1_lds_load(p, lds + t * 36);
+    g1_mul_dev(p, p, T->inv2k_std[LOGN]);
+    __syncthreads();
+    const unsigned dst = bitrev(t, LOGN);
+    g1_lds_store(lds + dst * 36, p);
+    g1_inf(p);
+    g1_lds_store(lds + (dst | 1) * 36, p);
+    __syncthreads();
+    g1_dit<false>(lds, LOGN, T);
+    for (unsigned e = t; e < n; e += half) {
+        g1_lds_load(p, lds + e * 36);
+        g1_gstore_std(h_fft + (size_t)e * G1P_WORDS, p);
+    }
+}
+
+// h[row][n/2..n) = (0, 1, 0)  (fk20_poly2h_fft.cu:27)
+__global__ void k_hext2h(uint32_t *h, unsigned rows, unsigned n2) {
+    const size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const unsigned hh = n2 / 2;
+    if (e >= (size_t)rows * hh) return;
+    const size_t row = e / hh, j = hh + e % hh;
+    uint4 *q = reinterpret_cast<uint4 *>(h + (row * n2 + j) * G1P_WORDS);
+#pragma unroll
+    for (int i = 0; i < 9; i++) q[i] = make_uint4(0, 0, 0, 0);
+    q[3] = make_uint4(1, 0, 0, 0);  // y[0] = 1 (word 12)
+}
+
+// Standard-form points -> packed Montgomery points (internal operand form).
+__global__ void k_std_to_packed(uint32_t *packed, const uint32_t *pts, size_t count) {
+    const size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= count) return;
+    g1p p;
+    g1_gload_std(p, pts + e * G1P_WORDS);
+    g1_lds_store(packed + e * 36, p);
+}
+
+// hext_fft[row][j] = sum_i tc_fft[row][i][j] * xext[i][j] (fk20_msm.cu:23-66), one thread per
+// (row, column).  The 16 scalar multiplications share one doubling chain (Straus).
+__global__ void __launch_bounds__(64) k_msm_plain(uint32_t *hext, const uint32_t *tc_fft,
+                                                  const uint32_t *xpk, unsigned n2) {
+    const unsigned j = blockIdx.x * 64 + threadIdx.x;
+    const size_t row = blockIdx.y;
+    if (j >= n2) return;
+    const uint32_t *tc = tc_fft + row * 16 * n2 * FR_WORDS;
+    g1p acc;
+    g1_inf(acc);
+#pragma unroll 1
+    for (int w = 7; w >= 0; w--) {
+        uint32_t kw[16];
+#pragma unroll
+        for (int i = 0; i < 16; i++) kw[i] = tc[((size_t)i * n2 + j) * FR_WORDS + w];
+#pragma unroll 1
+        for (int b = 31; b >= 0; b--) {
+            g1_dbl(acc, acc);
+#pragma unroll 1
+            for (int i = 0; i < 16; i++) {
+                if ((kw[i] >> b) & 1) {
+                    g1p X;
+                    g1_lds_load(X, xpk + ((size_t)i * n2 + j) * 36);
+                    g1_add(acc, acc, X);
+                }
+            }
+        }
+    }
+    g1_gstore_std(hext + (row * n2 + j) * G1P_WORDS, acc);
+}
+
+// xext[i][m] = srs[N-17-i-16m] for m <= k-2, infinity otherwise (FK20Py fk20_multi.py:82-85:
+// setup[n-l-1-i::-l] + [Z1], padded to 2k with Z1).
+__global__ void k_setup_gather(uint32_t *xext, const uint32_t *srs, unsigned N) {
+    const unsigned n2 = N / 8, k = N / 16;
+    const unsigned e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= 16 * n2) return;
+    const unsigned i = e / n2, m = e % n2;
+    uint4 *q = reinterpret_cast<uint4 *>(xext + (size_t)e * G1P_WORDS);
+    if (m + 2 <= k) {
+        const uint4 *src = reinterpret_cast<const uint4 *>(srs + (size_t)(N - 17 - i - 16 * m) * G1P_WORDS);
+#pragma unroll
+        for (int w = 0; w < 9; w++) q[w] = src[w];
+    } else {
+#pragma unroll
+        for (int w = 0; w < 9; w++) q[w] = make_uint4(0, 0, 0, 0);
+        q[3] = make_uint4(1, 0, 0, 0);
+    }
+}
+
+// Canonical affine (x, y); infinity -> (0, 0) as g1a_fromG1p (g1a.cu:48-56) gives.
+__global__ void k_to_affine(uint32_t *out, const uint32_t *in, size_t count) {
+    const size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= count) return;
+    g1p p;
+    g1_gload_std(p, in + e * G1P_WORDS);
+    fe x, y;
+    g1_to_affine(x, y, p);
+    fe_from_mont(x, x);
+    fe_from_mont(y, y);
+    uint32_t w[24];
+    fe_pack(w, x);
+    fe_pack(w + 12, y);
+    uint4 *q = reinterpret_cast<uint4 *>(out + e * G1A_WORDS);
+#pragma unroll
+    for (int i = 0; i < 6; i++) q[i] = make_uint4(w[4 * i], w[4 * i + 1], w[4 * i + 2], w[4 * i + 3]);
+}
+
+// ---------------------------------------------------------------- element-wise ops
+
+__device__ __forceinline__ void fe_gload_std(fe &a, const uint32_t *g) {
+    uint32_t w[12];
+#pragma unroll
+    for (int i = 0; i < 3; i++) {
+        uint4 v = reinterpret_cast<const uint4 *>(g)[i];
+        w[4 * i] = v.x; w[4 * i + 1] = v.y; w[4 * i + 2] = v.z; w[4 * i + 3] = v.w;
+    }
+    fe t;
+    fe_unpack(t, w);
+    fe_to_mont(a, t);
+}
+
+__device__ __forceinline__ void fe_gstore_std(uint32_t *g, const fe &a) {
+    fe t;
+    fe_from_mont(t, a);
+    uint32_t w[12];
+    fe_pack(w, t);
+#pragma unroll
+    for (int i = 0; i < 3; i++)
+        reinterpret_cast<uint4 *>(g)[i] = make_uint4(w[4 * i], w[4 * i + 1], w[4 * i + 2], w[4 * i + 3]);
+}
+
+__global__ void k_fp_op(int op, uint32_t *out, const uint32_t *a, const uint32_t *b, size_t count) {
+    const size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= count) return;
+    fe x, y, z;
+    fe_gload_std(x, a + e * 12);
+    fe_zero(y);
+    if (op <= 2 && b) fe_gload_std(y, b + e * 12);  // binary ops only
+    switch (op) {
+    case 0: fe_add(z, x, y); break;
+    case 1: fe_sub<2>(z, x, y); break;
+    case 2: fe_mul(z, x, y); break;
+    case 3: fe_sqr(z, x); break;
+    case 4: fe_neg<2>(z, x); break;
+    case 5: fe_inv(z, x); break;
+    default: z = x; break;
+    }
+    fe_gstore_std(out + e * 12, z);
+}
+
+__global__ void k_g1_op(int op, uint32_t *out, const uint32_t *pp, const uint32_t *qq, const uint32_t *kk,
+                        size_t count) {
+    const size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= count) return;
+    g1p p, q, r;
+    g1_gload_std(p, pp + e * G1P_WORDS);
+    switch (op) {
+    case 0: g1_gload_std(q, qq + e * G1P_WORDS); g1_add(r, p, q); break;
+    case 1: g1_gload_std(q, qq + e * G1P_WORDS); g1_neg(q, q); g1_add(r, p, q); break;
+    case 2: g1_dbl(r, p); break;
+    case 3: {
+        uint32_t k[8];
+#pragma unroll
+        for (int i = 0; i < 8; i++) k[i] = kk[e * FR_WORDS + i];
+        g1_mul_dev(r, p, k);
+    } break;
+    case 4: g1_neg(r, p); break;
+    case 5: {
+        fe ax, ay;
+        fe_gload_std(ax, qq + e * G1A_WORDS);
+        fe_gload_std(ay, qq + e * G1A_WORDS + 12);
+        fe cx, cy;
+        fe_canonical(cx, ax);
+        fe_canonical(cy, ay);
+        if (fe_is_zero_canonical(cx) && fe_is_zero_canonical(cy)) r = p;
+        else g1_add_aff(r, p, ax, ay);
+    } break;
+    case 6:
+    case 7: {
+        // 6: y^2 z == x^3 + 4 z^3 (or infinity); 7: cross-multiplied equality
+        bool res;
+        if (op == 6) {
+            fe l, rr, t, zc, xc, yc;
+            fe_canonical(zc, p.z);
+            fe_canonical(xc, p.x);
+            fe_canonical(yc, p.y);
+            if (fe_is_zero_canonical(zc)) {
+                res = fe_is_zero_canonical(xc) && !fe_is_zero_canonical(yc);
+            } else {
+                fe_sqr(l, p.y); fe_mul(l, l, p.z);
+                fe_sqr(rr, p.x); fe_mul(rr, rr, p.x);
+                fe_sqr(t, p.z); fe_mul(t, t, p.z); fe_mul_small<4>(t, t);
+                fe_add(rr, rr, t);
+                fe_sub<16>(l, l, rr);  // rr < 10p
+                fe_canonical(l, l);
+                res = fe_is_zero_canonical(l);
+            }
+        } else {
+            g1_gload_std(q, qq + e * G1P_WORDS);
+            fe za, zb, a1, b1;
+            fe_canonical(za, p.z);
+            fe_canonical(zb, q.z);
+            const bool ia = fe_is_zero_canonical(za), ib = fe_is_zero_canonical(zb);
+            if (ia || ib) {
+                res = ia && ib;
+            } else {
+                fe_mul(a1, p.x, q.z); fe_mul(b1, q.x, p.z);
+                fe_sub<2>(a1, a1, b1); fe_canonical(a1, a1);
+                res = fe_is_zero_canonical(a1);
+                fe_mul(a1, p.y, q.z); fe_mul(b1, q.y, p.z);
+                fe_sub<2>(a1, a1, b1); fe_canonical(a1, a1);
+                res = res && fe_is_zero_canonical(a1);
+            }
+        }
+        uint4 *o = reinterpret_cast<uint4 *>(out + e * G1P_WORDS);
+#pragma unroll
+        for (int i = 0; i < 9; i++) o[i] = make_uint4(0, 0, 0, 0);
+        o[0].x = res ? 1u : 0u;
+        return;
+    }
+    default: r = p; break;
+    }
+    g1_gstore_std(out + e * G1P_WORDS, r);
+}
+
+// ---------------------------------------------------------------- launchers
+
+template <bool INV>
+static hipError_t g1_fft_dispatch(uint32_t *out, const uint32_t *in, unsigned count, unsigned n,
+                                  const DevTables *T, hipStream_t s) {
+    switch (n) {
+    case 128: hipLaunchKernelGGL((k_g1_fft<7, INV>), dim3(count), dim3(64), 0, s, out, in, T); break;
+    case 256: hipLaunchKernelGGL((k_g1_fft<8, INV>), dim3(count), dim3(128), 0, s, out, in, T); break;
+    case 512: hipLaunchKernelGGL((k_g1_fft<9, INV>), dim3(count), dim3(256), 0, s, out, in, T); break;
+    default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_g1_fft(uint32_t *out, const uint32_t *in, unsigned count, unsigned n, bool inverse,
+                         const DevTables *T, hipStream_t s) {
+    if (!count) return hipSuccess;
+    return inverse ? g1_fft_dispatch<true>(out, in, count, n, T, s) : g1_fft_dispatch<false>(out, in, count, n, T, s);
+}
+
+hipError_t launch_hext_fft2h_fft(uint32_t *h_fft, const uint32_t *hext_fft, unsigned rows, unsigned n2,
+                                 const DevTables *T, hipStream_t s) {
+    if (!rows) return hipSuccess;
+    switch (n2) {
+    case 128: hipLaunchKernelGGL(k_hext_fft2h_fft<7>, dim3(rows), dim3(64), 0, s, h_fft, hext_fft, T); break;
+    case 256: hipLaunchKernelGGL(k_hext_fft2h_fft<8>, dim3(rows), dim3(128), 0, s, h_fft, hext_fft, T); break;
+    case 512: hipLaunchKernelGGL(k_hext_fft2h_fft<9>, dim3(rows), dim3(256), 0, s, h_fft, hext_fft, T); break;
+    default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_hext2h(uint32_t *h, unsigned rows, unsigned n2, hipStream_t s) {
+    const size_t total = (size_t)rows * (n2 / 2);
+    if (!total) return hipSuccess;
+    hipLaunchKernelGGL(k_hext2h, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, s, h, rows, n2);
+    return hipGetLastError();
+}
+
+hipError_t launch_std_to_packed(uint32_t *packed, const uint32_t *pts, size_t count, hipStream_t s) {
+    if (!count) return hipSuccess;
+    hipLaunchKernelGGL(k_std_to_packed, dim3((unsigned)((count + 63) / 64)), dim3(64), 0, s, packed, pts, count);
+    return hipGetLastError();
+}
+
+hipError_t launch_msm_plain(uint32_t *hext, const uint32_t *tc_fft, const uint32_t *xpk, unsigned rows,
+                            unsigned n2, hipStream_t s) {
+    if (!rows) return hipSuccess;
+    hipLaunchKernelGGL(k_msm_plain, dim3((n2 + 63) / 64, rows), dim3(64), 0, s, hext, tc_fft, xpk, n2);
+    return hipGetLastError();
+}
+
+hipError_t launch_setup_gather(uint32_t *xext, const uint32_t *srs, unsigned N, hipStream_t s) {
+    const unsigned total = 16 * (N / 8);
+    hipLaunchKernelGGL(k_setup_gather, dim3((total + 255) / 256), dim3(256), 0, s, xext, srs, N);
+    return hipGetLastError();
+}
+
+hipError_t launch_to_affine(uint32_t *out, const uint32_t *in, size_t count, hipStream_t s) {
+    if (!count) return hipSuccess;
+    hipLaunchKernelGGL(k_to_affine, dim3((unsigned)((count + 63) / 64)), dim3(64), 0, s, out, in, count);
+    return hipGetLastError();
+}
+
+hipError_t launch_fp_op(int op, uint32_t *out, const uint32_t *a, const uint32_t *b, size_t count,
+                        hipStream_t s) {
+    if (!count) return hipSuccess;
+    hipLaunchKernelGGL(k_fp_op, dim3((unsigned)((count + 63) / 64)), dim3(64), 0, s, op, out, a, b, count);
+    return hipGetLastError();
+}
+
+hipError_t launch_g1_op(int op, uint32_t *out, const uint32_t *p, const uint32_t *q, const uint32_t *k,
+                        size_t count, hipStream_t s) {
+    if (!count) return hipSuccess;
+    hipLaunchKernelGGL(k_g1_op, dim3((unsigned)((count + 63) / 64)), dim3(64), 0, s, op, out, p, q, k, count);
+    return hipGetLastError();
+}
+
+}  // namespace fk20

@@ -1,0 +1,146 @@
+/*
+ * fk20.h -- C ABI of the MI355X (gfx950) FK20 multi-proof library, libfk20.so.
+ *
+ * Drop-in replacement for the per-row FK20 path of lCardosoSantos/FK20-CUDA
+ * (fk20.cuh:20-38, fr.cuh:62-69, g1.cuh:83-100).  The reference exposes CUDA kernels
+ * with a launch-shape contract (grid = rows, 256 threads, dynamic shared memory) and
+ * void returns; here every entry point is a host function that takes device pointers,
+ * the row/transform count, the size, and a stream, and returns a status code.
+ *
+ * Types are byte-compatible with the reference ABI:
+ *   fk20_fr_t  = fr_t   (fr.cuh:24)   4 x u64 little endian, standard form
+ *   fk20_g1p_t = g1p_t  (g1.cuh:27)   x, y, z, each 6 x u64, homogeneous projective
+ *   fk20_g1a_t = g1a_t  (g1.cuh:20)   x, y affine, infinity = (0, 0) (g1a.cu:91)
+ * Inputs may be any representative (fr: any 256-bit value, fp: any 384-bit value).
+ * Outputs are standard form with every coordinate canonical (< p or < r); projective
+ * outputs are not normalised (use fk20_g1p_to_affine for canonical affine points).
+ * G1 inputs must be points of the order-r subgroup (all FK20 data are).
+ *
+ * Sizes: N is the polynomial length (1024, 2048 or 4096; the reference fixes 4096),
+ * l = 16 is fixed, k = N/16, and n2 = 2k = N/8 is the FFT size (128, 256, 512).
+ * All buffers are device memory; aliasing of input and output is allowed where the
+ * reference allows it (fr_fft.cu:21, g1p_fft.cu:17).  Calls are asynchronous on
+ * `stream` (a hipStream_t; NULL = the default stream) unless stated otherwise.
+ */
+#ifndef FK20_H
+#define FK20_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct { uint64_t v[4]; } fk20_fr_t;
+typedef struct { uint64_t x[6], y[6], z[6]; } fk20_g1p_t;
+typedef struct { uint64_t x[6], y[6]; } fk20_g1a_t;
+typedef void *fk20_stream_t; /* hipStream_t */
+
+enum {
+    FK20_OK = 0,
+    FK20_EINVAL = -1,   /* bad size, count or null pointer */
+    FK20_ENOMEM = -2,   /* device allocation failed or budget exceeded */
+    FK20_EHIP = -3,     /* HIP runtime error (fk20_last_hip_error gives the code) */
+    FK20_ENODEV = -4    /* no usable gfx950 device */
+};
+
+/* Last HIP error code seen by the library in this thread (0 if none). */
+int fk20_last_hip_error(void);
+const char *fk20_status_string(int status);
+
+/* ------------------------------------------------------------------ drop-in entry points */
+
+/* polynomial[rows][N] -> toeplitz_coefficients[rows][16][n2].
+ * Replaces fk20_poly2toeplitz_coefficients (fk20.cuh:25; fk20_poly2toeplitz_coefficients.cu:25). */
+int fk20_poly2toeplitz_coefficients(fk20_fr_t *toeplitz_coefficients, const fk20_fr_t *polynomial,
+                                    unsigned rows, unsigned N, fk20_stream_t stream);
+
+/* polynomial[rows][N] -> toeplitz_coefficients_fft[rows][16][n2] (Toeplitz + NTT fused).
+ * Replaces fk20_poly2toeplitz_coefficients_fft (fk20.cuh:26; fk20_poly2toeplitz_coefficients_fft.cu:24). */
+int fk20_poly2toeplitz_coefficients_fft(fk20_fr_t *toeplitz_coefficients_fft,
+                                        const fk20_fr_t *polynomial, unsigned rows, unsigned N,
+                                        fk20_stream_t stream);
+
+/* count NTTs of size n (128/256/512): output[i] = sum_j input[j] w^(ij), w = 5^((r-1)/n).
+ * Replaces fr_fft_wrapper (fr.cuh:67; fr_fft.cu:318) and fr_fft (fr.cuh:62). */
+int fr_fft(fk20_fr_t *output, const fk20_fr_t *input, unsigned count, unsigned n,
+           fk20_stream_t stream);
+/* Inverse NTT including the 1/n factor.  Replaces fr_ift_wrapper (fr.cuh:68; fr_fft.cu:347). */
+int fr_ift(fk20_fr_t *output, const fk20_fr_t *input, unsigned count, unsigned n,
+           fk20_stream_t stream);
+
+/* count G1 FFTs of size n.  Replaces g1p_fft_wrapper (g1.cuh:96; g1p_fft.cu:337). */
+int g1p_fft(fk20_g1p_t *output, const fk20_g1p_t *input, unsigned count, unsigned n,
+            fk20_stream_t stream);
+/* Inverse G1 FFT including 1/n.  Replaces g1p_ift_wrapper (g1.cuh:97; g1p_fft.cu:350). */
+int g1p_ift(fk20_g1p_t *output, const fk20_g1p_t *input, unsigned count, unsigned n,
+            fk20_stream_t stream);
+
+/* hext_fft[rows][n2] = sum_i tc_fft[rows][i][n2] * xext_fft[i][n2] (16 terms per column).
+ * Replaces fk20_msm (fk20.cuh:33; fk20_msm.cu:23). */
+int fk20_msm(fk20_g1p_t *hext_fft, const fk20_fr_t *toeplitz_coefficients_fft,
+             const fk20_g1p_t *xext_fft, unsigned rows, unsigned n2, fk20_stream_t stream);
+
+/* h[rows][n2]: upper half set to infinity in place.  Replaces fk20_hext2h (fk20.cuh:32;
+ * fk20_poly2h_fft.cu:27). */
+int fk20_hext2h(fk20_g1p_t *h, unsigned rows, unsigned n2, fk20_stream_t stream);
+
+/* h_fft = FFT(upper half zeroed(IFFT(hext_fft))) per row, fused.
+ * Replaces fk20_hext_fft2h_fft (fk20.cuh:31; fk20_hext_fft2h_fft.cu:27). */
+int fk20_hext_fft2h_fft(fk20_g1p_t *h_fft, const fk20_g1p_t *hext_fft, unsigned rows, unsigned n2,
+                        fk20_stream_t stream);
+
+/* The plain per-row launch sequence: polynomial[rows][N] + xext_fft[16][n2] -> h_fft[rows][n2].
+ * Replaces fk20_poly2h_fft (fk20.cuh:22; fk20_poly2h_fft.cu:47-80).  Like the reference it
+ * uses library-owned scratch (grown on demand, per device) and is not reentrant; use the
+ * context API below for concurrent or allocation-free use. */
+int fk20_poly2h_fft(fk20_g1p_t *h_fft, const fk20_fr_t *polynomial, const fk20_g1p_t *xext_fft,
+                    unsigned rows, unsigned N, fk20_stream_t stream);
+
+/* setup SRS [tau^i]G (i = 0..N) -> xext_fft[16][n2] by FK20Py toeplitz_part1.
+ * Replaces fk20_setup2xext_fft (fk20.cuh:20; fk20.cu:28), whose reference version is
+ * marked as failing its test (fk20.cu:29); this one follows FK20Py fk20_multi.py:82-85. */
+int fk20_setup2xext_fft(fk20_g1p_t *xext_fft, const fk20_g1p_t *setup, unsigned N,
+                        fk20_stream_t stream);
+
+/* ------------------------------------------------------------------ helpers */
+
+/* Canonical affine form of count points.  Replaces g1a_fromG1p_wrapper (g1.cuh:98; g1p.cu:168). */
+int fk20_g1p_to_affine(fk20_g1a_t *out, const fk20_g1p_t *in, size_t count, fk20_stream_t stream);
+
+/* Element-wise arithmetic over count elements, the counterparts of the reference's test
+ * wrappers (g1test_fft.cu:39 g1p_add_wrapper, :70 g1p_mul_wrapper; fptest/frtest kernels).
+ * fp ops: 0 add, 1 sub, 2 mul, 3 sqr, 4 neg(a), 5 inv(a), 6 reduce(a)
+ * fr ops: 0 add, 1 sub, 2 mul, 3 sqr, 4 neg(a), 5 inv(a), 6 reduce(a)
+ * g1 ops: 0 add(p,q), 1 sub(p,q), 2 dbl(p), 3 mul(p,k), 4 neg(p), 5 add_affine(p, q as g1a),
+ *         6 is_point(p) -> out x[0] = 0/1, 7 eq(p,q) -> out x[0] = 0/1 */
+int fk20_fp_op(int op, uint64_t *out, const uint64_t *a, const uint64_t *b, size_t count,
+               fk20_stream_t stream);
+int fk20_fr_op(int op, fk20_fr_t *out, const fk20_fr_t *a, const fk20_fr_t *b, size_t count,
+               fk20_stream_t stream);
+int fk20_g1p_op(int op, fk20_g1p_t *out, const fk20_g1p_t *p, const void *q, const fk20_fr_t *k,
+                size_t count, fk20_stream_t stream);
+
+/* Device memory and stream plumbing for hosts without a GPU framework (ctypes, cgo). */
+int fk20_device_count(int *count);
+int fk20_set_device(int device);
+int fk20_malloc(void **ptr, size_t bytes);
+int fk20_free(void *ptr);
+int fk20_memcpy_h2d(void *dst, const void *src, size_t bytes);
+int fk20_memcpy_d2h(void *dst, const void *src, size_t bytes);
+int fk20_memset(void *dst, int value, size_t bytes);
+int fk20_synchronize(void);
+int fk20_stream_create(fk20_stream_t *stream);
+int fk20_stream_destroy(fk20_stream_t stream);
+int fk20_stream_synchronize(fk20_stream_t stream);
+typedef void *fk20_event_t; /* hipEvent_t */
+int fk20_event_create(fk20_event_t *event);
+int fk20_event_destroy(fk20_event_t event);
+int fk20_event_record(fk20_event_t event, fk20_stream_t stream);
+int fk20_event_elapsed(float *ms, fk20_event_t start, fk20_event_t stop); /* waits for stop */
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,109 @@
+// Host build of the device arithmetic headers (fp.hip.h, fr.hip.h, g1.hip.h) for CPU tests:
+// the same source the HIP kernels inline, compiled by g++ with FK20_HD = inline.
+// Every function takes and returns the reference ABI layout (standard form, u64 limbs).
+#define FK20_HD inline
+#define __host__
+#define __device__
+#include "../../fk20-cuda_amd/csrc/g1.hip.h"
+
+#include <cstring>
+
+using namespace fk20;
+
+static void ld(fe &a, const uint64_t *x) {
+    uint32_t w[12];
+    memcpy(w, x, 48);
+    fe t;
+    fe_unpack(t, w);
+    fe_to_mont(a, t);
+}
+static void st(uint64_t *x, const fe &a) {
+    fe t;
+    fe_from_mont(t, a);
+    uint32_t w[12];
+    fe_pack(w, t);
+    memcpy(x, w, 48);
+}
+static void ldp(g1p &p, const uint64_t *x) {
+    ld(p.x, x);
+    ld(p.y, x + 6);
+    ld(p.z, x + 12);
+}
+static void stp(uint64_t *x, const g1p &p) {
+    st(x, p.x);
+    st(x + 6, p.y);
+    st(x + 12, p.z);
+}
+
+extern "C" {
+// op: 0 add, 1 sub, 2 mul, 3 sqr, 4 neg, 5 inv, 6 reduce; `reps` repeats z = op(z, b) to
+// exercise lazy bounds (add/sub results feed the next step without reduction).
+void host_fp_op(int op, uint64_t *z, const uint64_t *a, const uint64_t *b, int reps) {
+    fe x, y, r;
+    ld(x, a);
+    if (b) ld(y, b);
+    r = x;
+    for (int i = 0; i < reps; i++) {
+        switch (op) {
+        case 0: fe_add(r, r, y); break;
+        case 1: fe_sub<2>(r, r, y); break;
+        case 2: fe_mul(r, r, y); break;
+        case 3: fe_sqr(r, r); break;
+        case 4: fe_neg<2>(r, r); break;
+        case 5: fe_inv(r, r); break;
+        default: break;
+        }
+    }
+    st(z, r);
+}
+
+void host_fr_mul(uint64_t *z, const uint64_t *a, const uint64_t *b) {
+    fs x, y;
+    memcpy(x.l, a, 32);
+    memcpy(y.l, b, 32);
+    fs_to_mont(x, x);
+    fs_to_mont(y, y);
+    fs_mul(x, x, y);
+    fs_from_mont(x, x);
+    memcpy(z, x.l, 32);
+}
+
+void host_g1_add(uint64_t *r, const uint64_t *p, const uint64_t *q) {
+    g1p a, b;
+    ldp(a, p);
+    ldp(b, q);
+    g1_add(a, a, b);
+    stp(r, a);
+}
+
+void host_g1_dbl(uint64_t *r, const uint64_t *p) {
+    g1p a;
+    ldp(a, p);
+    g1_dbl(a, a);
+    stp(r, a);
+}
+
+void host_g1_mul(uint64_t *r, const uint64_t *p, const uint64_t *k) {
+    g1p a;
+    ldp(a, p);
+    uint32_t kw[8];
+    memcpy(kw, k, 32);
+    g1_mul(a, a, kw);
+    stp(r, a);
+}
+
+// Chain of mixed operations that keeps intermediate values in the lazy (unreduced) range.
+void host_g1_chain(uint64_t *r, const uint64_t *p, int steps) {
+    g1p a, b;
+    ldp(a, p);
+    b = a;
+    for (int i = 0; i < steps; i++) {
+        g1_dbl(a, a);
+        g1_add(a, a, b);
+        g1p n;
+        g1_neg(n, b);
+        g1_add(b, a, n);
+    }
+    stp(r, b);
+}
+}

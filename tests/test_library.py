@@ -1,0 +1,61 @@
+"""CPU checks of libfk20.so: it loads, exports every function include/fk20.h declares, and
+rejects invalid arguments before touching a device."""
+import ctypes
+import os
+import re
+
+import pytest
+
+from fk20_amd import lib as L
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def header_functions():
+    text = open(os.path.join(ROOT, "include", "fk20.h")).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"^\s*(?:int|const char \*)\s*\**\s*(\w+)\s*\(", text, flags=re.M)))
+
+
+def test_header_parsed():
+    names = header_functions()
+    assert "fk20_poly2h_fft" in names and "fr_fft" in names and "g1p_ift" in names
+    assert len(names) >= 30
+
+
+def test_library_exports_every_declared_symbol():
+    lib = ctypes.CDLL(L.LIBPATH)
+    missing = [n for n in header_functions() if not hasattr(lib, n)]
+    assert not missing, missing
+
+
+def test_binding_covers_header():
+    assert set(header_functions()) - {"fk20_status_string"} <= set(L.EXPORTS)
+
+
+@pytest.mark.parametrize("call", [
+    lambda l: l.fk20_poly2toeplitz_coefficients(1, 1, 1, 1000, None),
+    lambda l: l.fk20_poly2toeplitz_coefficients(None, 1, 1, 4096, None),
+    lambda l: l.fk20_poly2toeplitz_coefficients_fft(1, 1, 1, 512, None),
+    lambda l: l.fr_fft(1, 1, 1, 64, None),
+    lambda l: l.fr_ift(None, 1, 1, 512, None),
+    lambda l: l.g1p_fft(1, 1, 1, 1024, None),
+    lambda l: l.g1p_ift(1, None, 2, 512, None),
+    lambda l: l.fk20_msm(1, 1, None, 1, 512, None),
+    lambda l: l.fk20_msm(1, 1, 1, 1, 100, None),
+    lambda l: l.fk20_hext2h(None, 1, 512, None),
+    lambda l: l.fk20_hext_fft2h_fft(1, 1, 1, 300, None),
+    lambda l: l.fk20_poly2h_fft(1, 1, 1, 1, 4000, None),
+    lambda l: l.fk20_poly2h_fft(1, None, 1, 1, 4096, None),
+    lambda l: l.fk20_setup2xext_fft(None, 1, 4096, None),
+    lambda l: l.fk20_fp_op(9, 1, 1, 1, 1, None),
+    lambda l: l.fk20_fr_op(0, 1, 1, None, 1, None),
+    lambda l: l.fk20_g1p_op(3, 1, 1, None, None, 1, None),
+])
+def test_invalid_arguments_rejected(call):
+    assert call(L.lib()) == -1  # FK20_EINVAL
+
+
+def test_zero_rows_is_a_no_op():
+    assert L.lib().fk20_poly2toeplitz_coefficients(None, None, 0, 4096, None) == 0
+    assert L.lib().fk20_hext2h(None, 0, 512, None) == 0
