@@ -44,7 +44,9 @@ struct DeviceState {
 std::mutex g_mu;
 std::vector<DeviceState> g_dev;
 
-void host_tables(DevTables &T) {
+}  // namespace
+
+void fk20::build_host_tables(DevTables &T) {
     // w = 5^((r-1)/512) in Montgomery form, by square-and-multiply on the host copy of
     // the device Fr arithmetic
     fs five, w, acc;
@@ -94,6 +96,8 @@ void host_tables(DevTables &T) {
     }
 }
 
+namespace {
+
 // Returns the state of the current device, creating its tables on first use.
 int device_state(DeviceState **out) {
     int dev = 0;
@@ -103,7 +107,7 @@ int device_state(DeviceState **out) {
     DeviceState &st = g_dev[dev];
     if (!st.tables) {
         DevTables h;
-        host_tables(h);
+        build_host_tables(h);
         void *d = nullptr;
         FK20_TRY(hipMalloc(&d, sizeof(DevTables)));
         hipError_t e = hipMemcpy(d, &h, sizeof(DevTables), hipMemcpyHostToDevice);
@@ -138,6 +142,20 @@ int device_scratch(DeviceState *st, size_t bytes, void **out) {
 uint32_t *W(const void *p) { return reinterpret_cast<uint32_t *>(const_cast<void *>(p)); }
 
 }  // namespace
+
+namespace fk20 {
+int status_from_hip(hipError_t e) { return hip_status(e); }
+
+const DevTables *current_device_tables(hipError_t *err) {
+    DeviceState *st;
+    *err = hipSuccess;
+    if (device_state(&st) != FK20_OK) {
+        *err = (hipError_t)(g_last_hip_error ? g_last_hip_error : (int)hipErrorUnknown);
+        return nullptr;
+    }
+    return st->tables;
+}
+}  // namespace fk20
 
 extern "C" {
 

@@ -31,6 +31,16 @@ __host__ __device__ __forceinline__ unsigned bitrev(unsigned x, unsigned bits) {
 #endif
 }
 
+// Host copy of the constant tables (api.cpp).
+void build_host_tables(DevTables &T);
+
+// FK20_* status of a HIP error, recorded as the thread's last HIP error (api.cpp).
+int status_from_hip(hipError_t e);
+
+// Constant tables of the current device, built on first use (api.cpp).  Returns nullptr
+// and sets *err on failure.
+const DevTables *current_device_tables(hipError_t *err);
+
 // ---- launchers (one per kernel family); all asynchronous on `s` ----
 hipError_t launch_toeplitz(uint32_t *tc, const uint32_t *poly, unsigned rows, unsigned N, hipStream_t s);
 hipError_t launch_toeplitz_fft(uint32_t *tc_fft, const uint32_t *poly, unsigned rows, unsigned N,

@@ -1,0 +1,357 @@
+// Context API of the batched FK20 path: resident setups (window tables), twiddle programs,
+// workspace and optional hipGraph replay.  See include/fk20.h for the interface.
+#include <hip/hip_runtime.h>
+
+#include <cstring>
+#include <mutex>
+#include <vector>
+
+#include "../../include/fk20.h"
+#include "fast.hip.h"
+
+using namespace fk20;
+
+namespace {
+
+typedef unsigned __int128 u128;
+
+// ---------------------------------------------------------------- twiddle programs
+
+// k = q * lambda + rem for a 256-bit k (8 words); q, rem < 2^128 because k < 2^255 and
+// lambda > 2^127.
+void divmod_lambda(const uint32_t *k, u128 &q, u128 &rem) {
+    const u128 lam = ((u128)GLV_LAMBDA[1] << 64) | GLV_LAMBDA[0];
+    u128 r = 0;
+    q = 0;
+    for (int i = 255; i >= 0; i--) {
+        const bool top = (r >> 127) != 0;
+        r = (r << 1) | ((k[i / 32] >> (i % 32)) & 1u);
+        q <<= 1;
+        if (top || r >= lam) {
+            r -= lam;
+            q |= 1;
+        }
+    }
+    rem = r;
+}
+
+// width-W non-adjacent form, least significant digit first; digits odd in (-2^(W-1), 2^(W-1))
+std::vector<int> wnaf(u128 k, int W) {
+    std::vector<int> d;
+    const int full = 1 << W, half = 1 << (W - 1);
+    while (k != 0) {
+        int digit = 0;
+        if (k & 1) {
+            digit = (int)(k & (u128)(full - 1));
+            if (digit >= half) digit -= full;
+            if (digit > 0) k -= (u128)digit;
+            else k += (u128)(-digit);
+        }
+        d.push_back(digit);
+        k >>= 1;
+    }
+    return d;
+}
+
+void append_program(std::vector<uint32_t> &ops, const uint32_t *k, int W) {
+    u128 k2, k1;
+    divmod_lambda(k, k2, k1);
+    const std::vector<int> d1 = wnaf(k1, W), d2 = wnaf(k2, W);
+    const int L = (int)std::max(d1.size(), d2.size());
+    bool started = false;
+    uint32_t pending = 0;
+    for (int t = L - 1; t >= 0; t--) {
+        if (started) pending++;
+        for (int part = 0; part < 2; part++) {
+            const std::vector<int> &d = part ? d2 : d1;
+            const int digit = t < (int)d.size() ? d[t] : 0;
+            if (!digit) continue;
+            const uint32_t e = (uint32_t)((digit < 0 ? -digit : digit) - 1) / 2;
+            ops.push_back(pending | (part ? TWOP_PHI : 0u) | (e << TWOP_ENTRY_SHIFT) | (digit < 0 ? TWOP_NEG : 0u));
+            pending = 0;
+            started = true;
+        }
+    }
+    ops.push_back(TWOP_END | pending);
+}
+
+// Evaluates a program on the scalar 1 with phi = lambda (Montgomery Fr): must give k.
+bool check_program(const uint32_t *ops, const fs &expect_mont) {
+    fs lam, acc, one;
+    fs_zero(lam);
+    lam.l[0] = (uint32_t)GLV_LAMBDA[0];
+    lam.l[1] = (uint32_t)(GLV_LAMBDA[0] >> 32);
+    lam.l[2] = (uint32_t)GLV_LAMBDA[1];
+    lam.l[3] = (uint32_t)(GLV_LAMBDA[1] >> 32);
+    fs_to_mont(lam, lam);
+    fs_set(one, FR_ONE);
+    fs_zero(acc);
+    for (int i = 0;; i++) {
+        const uint32_t op = ops[i];
+        for (uint32_t d = 0; d < (op & TWOP_DBL_MASK); d++) fs_add(acc, acc, acc);
+        if (op & TWOP_END) break;
+        fs term, m;
+        fs_zero(m);
+        m.l[0] = 2 * ((op >> TWOP_ENTRY_SHIFT) & 7) + 1;
+        fs_to_mont(term, m);
+        if (op & TWOP_PHI) fs_mul(term, term, lam);
+        if (op & TWOP_NEG) fs_sub(acc, acc, term);
+        else fs_add(acc, acc, term);
+    }
+    return memcmp(acc.l, expect_mont.l, 32) == 0;
+}
+
+struct Programs {
+    uint32_t *ops = nullptr;    // device
+    uint32_t *index = nullptr;  // device, [2][512]
+};
+
+// Builds the programs of w^idx and w^-idx (w of order 512) and checks each on the host.
+bool build_programs_host(std::vector<uint32_t> &ops, std::vector<uint32_t> &index) {
+    DevTables T;
+    build_host_tables(T);
+    ops.clear();
+    index.assign(2 * 512, 0);
+    for (int dir = 0; dir < 2; dir++) {
+        for (int idx = 0; idx < 512; idx++) {
+            const int e = dir == TW_FWD ? idx : (512 - idx) % 512;
+            index[dir * 512 + idx] = (uint32_t)ops.size();
+            append_program(ops, T.tw_std[e], TW_WNAF);
+            fs expect;
+            fs_set(expect, T.tw_mont[e]);
+            if (!check_program(ops.data() + index[dir * 512 + idx], expect)) return false;
+        }
+    }
+    return true;
+}
+
+hipError_t build_programs(Programs &P) {
+    std::vector<uint32_t> ops, index;
+    if (!build_programs_host(ops, index)) return hipErrorInvalidValue;
+    hipError_t err = hipMalloc(&P.ops, ops.size() * 4);
+    if (err == hipSuccess) err = hipMalloc(&P.index, index.size() * 4);
+    if (err == hipSuccess) err = hipMemcpy(P.ops, ops.data(), ops.size() * 4, hipMemcpyHostToDevice);
+    if (err == hipSuccess) err = hipMemcpy(P.index, index.data(), index.size() * 4, hipMemcpyHostToDevice);
+    return err;
+}
+
+// ---------------------------------------------------------------- context
+
+struct Setup {
+    bool live = false;
+    unsigned N = 0, n2 = 0, win = 0, nwin = 0;
+    uint32_t *table = nullptr;
+    uint32_t *inf = nullptr;
+};
+
+bool valid_N(unsigned N) { return N == 1024 || N == 2048 || N == 4096; }
+bool valid_win(unsigned w) { return w == 8 || w == 10 || w == 12; }
+unsigned nwin_of(unsigned w) { return (256 + w - 1) / w; }
+
+int status_of(hipError_t e) { return status_from_hip(e); }
+
+}  // namespace
+
+struct fk20_ctx {
+    int device = 0;
+    const DevTables *T = nullptr;
+    Programs prog;
+    std::vector<Setup> setups;
+    void *work = nullptr;
+    size_t work_bytes = 0;
+    bool use_graph = false;
+    struct GraphKey {
+        int setup;
+        unsigned rows;
+        const void *poly;
+        void *out;
+        hipStream_t stream;
+    } gkey{};
+    hipGraphExec_t gexec = nullptr;
+    std::mutex mu;
+};
+
+extern "C" {
+
+size_t fk20_setup_table_bytes(unsigned N, unsigned window) {
+    if (!valid_N(N) || !valid_win(window)) return 0;
+    return (size_t)16 * (N / 8) * nwin_of(window) * ((size_t)1 << (window - 1)) * 96;
+}
+
+size_t fk20_rows_workspace_bytes(unsigned rows, unsigned N) {
+    if (!valid_N(N)) return 0;
+    const size_t n2 = N / 8;
+    return (size_t)rows * n2 * (16 * 32 + 144);
+}
+
+int fk20_ctx_create(fk20_ctx_t **out) {
+    if (!out) return FK20_EINVAL;
+    fk20_ctx *c = new fk20_ctx();
+    hipError_t e = hipGetDevice(&c->device);
+    if (e == hipSuccess) c->T = current_device_tables(&e);
+    if (e == hipSuccess) e = build_programs(c->prog);
+    if (e != hipSuccess) {
+        fk20_ctx_destroy(c);
+        return status_of(e);
+    }
+    *out = c;
+    return FK20_OK;
+}
+
+int fk20_ctx_destroy(fk20_ctx_t *c) {
+    if (!c) return FK20_EINVAL;
+    (void)hipDeviceSynchronize();
+    for (Setup &s : c->setups) {
+        if (s.table) (void)hipFree(s.table);
+        if (s.inf) (void)hipFree(s.inf);
+    }
+    if (c->work) (void)hipFree(c->work);
+    if (c->prog.ops) (void)hipFree(c->prog.ops);
+    if (c->prog.index) (void)hipFree(c->prog.index);
+    if (c->gexec) (void)hipGraphExecDestroy(c->gexec);
+    delete c;
+    return FK20_OK;
+}
+
+int fk20_ctx_set_graph(fk20_ctx_t *c, int enable) {
+    if (!c) return FK20_EINVAL;
+    std::lock_guard<std::mutex> lk(c->mu);
+    c->use_graph = enable != 0;
+    return FK20_OK;
+}
+
+int fk20_setup_load(fk20_ctx_t *c, const fk20_g1p_t *xext_fft, unsigned N, unsigned window, int *setup_id,
+                    fk20_stream_t stream) {
+    if (!c || !xext_fft || !setup_id || !valid_N(N) || !valid_win(window)) return FK20_EINVAL;
+    std::lock_guard<std::mutex> lk(c->mu);
+    hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+    Setup st;
+    st.N = N;
+    st.n2 = N / 8;
+    st.win = window;
+    st.nwin = nwin_of(window);
+    const unsigned nbases = 16 * st.n2, H = 1u << (window - 1);
+    const unsigned logn2 = st.n2 == 128 ? 7 : st.n2 == 256 ? 8 : 9;
+    uint32_t *xs = nullptr, *zt = nullptr;
+    // bases per build launch, bounding the batch-inversion scratch to ~512 MiB
+    const size_t per_base = (size_t)st.nwin * H * 96;
+    unsigned chunk = (unsigned)std::max<size_t>(1, ((size_t)512 << 20) / per_base);
+    if (chunk > nbases) chunk = nbases;
+    hipError_t e = hipMalloc(&st.table, fk20_setup_table_bytes(N, window));
+    if (e == hipSuccess) e = hipMalloc(&st.inf, nbases * 4);
+    if (e == hipSuccess) e = hipMalloc(&xs, (size_t)nbases * 144);
+    if (e == hipSuccess) e = hipMalloc(&zt, (size_t)chunk * per_base);
+    if (e == hipSuccess)
+        e = launch_scale_bases(xs, st.inf, reinterpret_cast<const uint32_t *>(xext_fft), nbases,
+                               c->T->inv2k_std[logn2], s);
+    for (unsigned b0 = 0; e == hipSuccess && b0 < nbases; b0 += chunk)
+        e = launch_table_build(st.table, zt, xs, st.inf, b0, std::min(chunk, nbases - b0), window, st.nwin, s);
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    if (xs) (void)hipFree(xs);
+    if (zt) (void)hipFree(zt);
+    if (e != hipSuccess) {
+        if (st.table) (void)hipFree(st.table);
+        if (st.inf) (void)hipFree(st.inf);
+        return status_of(e);
+    }
+    st.live = true;
+    c->setups.push_back(st);
+    *setup_id = (int)c->setups.size() - 1;
+    return FK20_OK;
+}
+
+int fk20_setup_unload(fk20_ctx_t *c, int id) {
+    if (!c) return FK20_EINVAL;
+    std::lock_guard<std::mutex> lk(c->mu);
+    if (id < 0 || id >= (int)c->setups.size() || !c->setups[id].live) return FK20_EINVAL;
+    (void)hipDeviceSynchronize();
+    Setup &s = c->setups[id];
+    (void)hipFree(s.table);
+    (void)hipFree(s.inf);
+    s = Setup();
+    if (c->gexec && c->gkey.setup == id) {
+        (void)hipGraphExecDestroy(c->gexec);
+        c->gexec = nullptr;
+    }
+    return FK20_OK;
+}
+
+static hipError_t grow_work(fk20_ctx *c, size_t bytes) {
+    if (c->work_bytes >= bytes) return hipSuccess;
+    if (c->work) {
+        hipError_t e = hipDeviceSynchronize();
+        if (e != hipSuccess) return e;
+        (void)hipFree(c->work);
+        c->work = nullptr;
+        c->work_bytes = 0;
+        if (c->gexec) {
+            (void)hipGraphExecDestroy(c->gexec);
+            c->gexec = nullptr;
+        }
+    }
+    hipError_t e = hipMalloc(&c->work, bytes);
+    if (e == hipSuccess) c->work_bytes = bytes;
+    return e;
+}
+
+int fk20_ctx_reserve(fk20_ctx_t *c, unsigned rows, unsigned N) {
+    if (!c || !valid_N(N)) return FK20_EINVAL;
+    std::lock_guard<std::mutex> lk(c->mu);
+    return status_of(grow_work(c, fk20_rows_workspace_bytes(rows, N)));
+}
+
+static hipError_t enqueue_rows(fk20_ctx *c, const Setup &st, uint32_t *h_fft, const uint32_t *poly,
+                               unsigned rows, hipStream_t s) {
+    const unsigned n2 = st.n2;
+    uint32_t *tcT = reinterpret_cast<uint32_t *>(c->work);
+    uint32_t *D = tcT + (size_t)rows * 16 * n2 * 8;
+    hipError_t e = launch_toeplitz_fft_T(tcT, poly, rows, st.N, c->T, s);
+    if (e == hipSuccess) e = launch_msm_comb(D, tcT, st.table, st.inf, rows, n2, st.win, s);
+    if (e == hipSuccess) e = launch_g1_transforms(D, rows, n2, c->prog.ops, c->prog.index, s);
+    if (e == hipSuccess) e = launch_fft_output(h_fft, D, rows, n2, s);
+    return e;
+}
+
+int fk20_prove_rows(fk20_ctx_t *c, int id, fk20_g1p_t *h_fft, const fk20_fr_t *poly, unsigned rows,
+                    fk20_stream_t stream) {
+    if (!c) return FK20_EINVAL;
+    std::lock_guard<std::mutex> lk(c->mu);
+    if (id < 0 || id >= (int)c->setups.size() || !c->setups[id].live) return FK20_EINVAL;
+    if (!rows) return FK20_OK;
+    if (!h_fft || !poly) return FK20_EINVAL;
+    const Setup &st = c->setups[id];
+    hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+    hipError_t e = grow_work(c, fk20_rows_workspace_bytes(rows, st.N));
+    if (e != hipSuccess) return status_of(e);
+    uint32_t *out = reinterpret_cast<uint32_t *>(h_fft);
+    const uint32_t *in = reinterpret_cast<const uint32_t *>(poly);
+    if (!c->use_graph || s == nullptr) return status_of(enqueue_rows(c, st, out, in, rows, s));
+    // hipGraph: capture the launch sequence once per (setup, rows, buffers, stream), replay after
+    const fk20_ctx::GraphKey key{id, rows, poly, h_fft, s};
+    if (!c->gexec || memcmp(&key, &c->gkey, sizeof key) != 0) {
+        if (c->gexec) {
+            (void)hipGraphExecDestroy(c->gexec);
+            c->gexec = nullptr;
+        }
+        hipGraph_t g = nullptr;
+        e = hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal);
+        if (e != hipSuccess) return status_of(e);
+        hipError_t le = enqueue_rows(c, st, out, in, rows, s);
+        e = hipStreamEndCapture(s, &g);
+        if (le != hipSuccess) e = le;
+        if (e == hipSuccess) e = hipGraphInstantiate(&c->gexec, g, nullptr, nullptr, 0);
+        if (g) (void)hipGraphDestroy(g);
+        if (e != hipSuccess) return status_of(e);
+        c->gkey = key;
+    }
+    return status_of(hipGraphLaunch(c->gexec, s));
+}
+
+int fk20_twiddle_selfcheck(unsigned *total_ops) {
+    std::vector<uint32_t> ops, index;
+    const bool ok = build_programs_host(ops, index);
+    if (total_ops) *total_ops = (unsigned)ops.size();
+    return ok ? FK20_OK : FK20_EINVAL;
+}
+
+}  // extern "C"

@@ -47,6 +47,17 @@ _SIGS = {
     "fk20_fp_op": ([_I, _P, _P, _P, _SZ, _P], _I),
     "fk20_fr_op": ([_I, _P, _P, _P, _SZ, _P], _I),
     "fk20_g1p_op": ([_I, _P, _P, _P, _P, _SZ, _P], _I),
+    "fk20_ctx_create": ([ctypes.POINTER(_P)], _I),
+    "fk20_ctx_destroy": ([_P], _I),
+    "fk20_ctx_set_graph": ([_P, _I], _I),
+    "fk20_setup_table_bytes": ([_U, _U], _SZ),
+    "fk20_setup_load": ([_P, _P, _U, _U, ctypes.POINTER(_I), _P], _I),
+    "fk20_setup_unload": ([_P, _I], _I),
+    "fk20_rows_workspace_bytes": ([_U, _U], _SZ),
+    "fk20_ctx_reserve": ([_P, _U, _U], _I),
+    "fk20_prove_rows": ([_P, _I, _P, _P, _U, _P], _I),
+    "fk20_twiddle_selfcheck": ([ctypes.POINTER(_U)], _I),
+    "fk20_status_string": ([_I], ctypes.c_char_p),
     "fk20_device_count": ([ctypes.POINTER(_I)], _I),
     "fk20_set_device": ([_I], _I),
     "fk20_malloc": ([ctypes.POINTER(_P), _SZ], _I),

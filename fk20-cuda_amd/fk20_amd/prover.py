@@ -1,0 +1,78 @@
+"""Batched FK20 prover: the context API of libfk20 (fk20_ctx_*, fk20_setup_load,
+fk20_prove_rows) with device buffers managed from Python."""
+import ctypes
+
+import numpy as np
+
+from .lib import DeviceBuffer, call, lib, synchronize
+
+
+class Prover:
+    """One libfk20 context on the current device; setups stay resident until closed."""
+
+    def __init__(self, graph=False):
+        c = ctypes.c_void_p()
+        call("fk20_ctx_create", ctypes.byref(c))
+        self.ctx = c.value
+        self.setups = {}
+        if graph:
+            call("fk20_ctx_set_graph", self.ctx, 1)
+
+    def load_setup(self, xext_fft, N, window=8, stream=None):
+        """xext_fft: host array (16, N/8, 18) or a DeviceBuffer holding it."""
+        buf = xext_fft if isinstance(xext_fft, DeviceBuffer) else DeviceBuffer.from_array(
+            np.ascontiguousarray(xext_fft, dtype=np.uint64))
+        sid = ctypes.c_int()
+        call("fk20_setup_load", self.ctx, buf.ptr, N, window, ctypes.byref(sid), stream)
+        if buf is not xext_fft:
+            buf.free()
+        self.setups[sid.value] = N
+        return sid.value
+
+    def unload_setup(self, sid):
+        call("fk20_setup_unload", self.ctx, sid)
+        self.setups.pop(sid, None)
+
+    def reserve(self, rows, N):
+        call("fk20_ctx_reserve", self.ctx, rows, N)
+
+    def prove_device(self, sid, h_fft, poly, rows, stream=None):
+        """Enqueue rows on stream: poly and h_fft are DeviceBuffers (or raw pointers)."""
+        hp = h_fft.ptr if isinstance(h_fft, DeviceBuffer) else h_fft
+        pp = poly.ptr if isinstance(poly, DeviceBuffer) else poly
+        call("fk20_prove_rows", self.ctx, sid, hp, pp, rows, stream)
+
+    def prove(self, sid, poly):
+        """Host polynomials (rows, N, 4) -> host h_fft (rows, N/8, 18)."""
+        poly = np.ascontiguousarray(poly, dtype=np.uint64)
+        rows, N = poly.shape[0], poly.shape[1]
+        assert self.setups[sid] == N
+        pb = DeviceBuffer.from_array(poly)
+        hb = DeviceBuffer(rows * (N // 8) * 144)
+        self.prove_device(sid, hb, pb, rows)
+        synchronize()
+        out = hb.download((rows, N // 8, 18))
+        pb.free()
+        hb.free()
+        return out
+
+    def close(self):
+        if self.ctx:
+            call("fk20_ctx_destroy", self.ctx)
+            self.ctx = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+
+def setup_table_bytes(N, window):
+    return int(lib().fk20_setup_table_bytes(N, window))
+
+
+def twiddle_selfcheck():
+    n = ctypes.c_uint()
+    status = lib().fk20_twiddle_selfcheck(ctypes.byref(n))
+    return status, n.value

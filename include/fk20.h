@@ -104,6 +104,43 @@ int fk20_poly2h_fft(fk20_g1p_t *h_fft, const fk20_fr_t *polynomial, const fk20_g
 int fk20_setup2xext_fft(fk20_g1p_t *xext_fft, const fk20_g1p_t *setup, unsigned N,
                         fk20_stream_t stream);
 
+/* ------------------------------------------------------------------ batched path (context API)
+ *
+ * The reference's fast 512-row path (fk20benchmark.cu:287-309: fk20_msm_makelut once per
+ * setup, then fk20_msm_comb and the graph-captured G1 FFT) as a context that keeps each
+ * setup's tables resident and owns its workspace, so proving allocates nothing once the
+ * workspace is reserved.  A context is bound to the device current at creation; calls on
+ * one context are serialised by the library.  No reference counterpart exists for the
+ * context type itself; fk20_setup_load replaces fk20_msm_makelut (fk20.cuh:34;
+ * fk20_msm.cu:79) and fk20_prove_rows replaces the fk20_poly2h_fft sequence with
+ * fk20_msm_comb (fk20.cuh:35) and fk20_hext_fft_2_h_fft_512 (fk20.cuh:38). */
+typedef struct fk20_ctx fk20_ctx_t;
+
+int fk20_ctx_create(fk20_ctx_t **ctx);
+int fk20_ctx_destroy(fk20_ctx_t *ctx);
+/* 1: fk20_prove_rows captures its launch sequence into a hipGraph per (setup, rows,
+ * buffers, stream) and replays it (needs a non-NULL stream); 0: direct launches. */
+int fk20_ctx_set_graph(fk20_ctx_t *ctx, int enable);
+/* Device bytes of one setup's window tables for polynomial length N and window width w
+ * (8, 10 or 12 bits): 16 * (N/8) * ceil(256/w) * 2^(w-1) * 96. */
+size_t fk20_setup_table_bytes(unsigned N, unsigned window);
+/* Builds the resident tables of xext_fft[16][N/8] (device memory, standard form);
+ * synchronises `stream`.  The returned id names the setup in fk20_prove_rows. */
+int fk20_setup_load(fk20_ctx_t *ctx, const fk20_g1p_t *xext_fft, unsigned N, unsigned window,
+                    int *setup_id, fk20_stream_t stream);
+int fk20_setup_unload(fk20_ctx_t *ctx, int setup_id);
+/* Workspace bytes fk20_prove_rows needs for `rows` rows of length N. */
+size_t fk20_rows_workspace_bytes(unsigned rows, unsigned N);
+/* Grows the context workspace to fit `rows` rows of length N. */
+int fk20_ctx_reserve(fk20_ctx_t *ctx, unsigned rows, unsigned N);
+/* polynomial[rows][N] -> h_fft[rows][N/8] for the setup's N: Toeplitz + NTT, window-table
+ * MSM, column-batched G1 IFFT / zero / FFT with endomorphism-split twiddles. */
+int fk20_prove_rows(fk20_ctx_t *ctx, int setup_id, fk20_g1p_t *h_fft, const fk20_fr_t *polynomial,
+                    unsigned rows, fk20_stream_t stream);
+/* Host-only check of the twiddle programs (GLV split + wNAF of every w^+-i, i < 512,
+ * evaluated over Fr); total_ops receives the program size.  Needs no device. */
+int fk20_twiddle_selfcheck(unsigned *total_ops);
+
 /* ------------------------------------------------------------------ helpers */
 
 /* Canonical affine form of count points.  Replaces g1a_fromG1p_wrapper (g1.cuh:98; g1p.cu:168). */

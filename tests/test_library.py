@@ -59,3 +59,19 @@ def test_invalid_arguments_rejected(call):
 def test_zero_rows_is_a_no_op():
     assert L.lib().fk20_poly2toeplitz_coefficients(None, None, 0, 4096, None) == 0
     assert L.lib().fk20_hext2h(None, 0, 512, None) == 0
+
+
+def test_twiddle_programs_selfcheck():
+    """GLV split + wNAF programs of every w^(+-i), i < 512, evaluate to the root over Fr."""
+    from fk20_amd import prover
+    status, nops = prover.twiddle_selfcheck()
+    assert status == 0
+    # about 2 x 26 additions per program (wNAF-4 digits of two 128-bit halves)
+    assert 1024 * 40 < nops < 1024 * 70
+
+
+def test_setup_table_bytes():
+    from fk20_amd import prover
+    assert prover.setup_table_bytes(4096, 8) == 16 * 512 * 32 * 128 * 96
+    assert prover.setup_table_bytes(1024, 10) == 16 * 128 * 26 * 512 * 96
+    assert prover.setup_table_bytes(4096, 9) == 0
