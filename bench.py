@@ -4,15 +4,18 @@
 A step is one pass of the FK20 path over one batch of synthetic rows (uniform Fr
 coefficients, one seeded setup): polynomial -> Toeplitz coefficients -> 16 Fr NTTs per
 row -> MSM -> G1 IFFT -> zero upper half -> G1 FFT -> h_fft.  Inputs are resident in HBM
-before timing starts.  Forms:
-  plain  (P2) the reference's six-launch sequence with grid = rows (fk20_poly2h_fft.cu:47-80)
-  P1          the same sequence issued once per row (rows = 1), timed on a bounded sample
-The dominant kernel is timed with HIP events on its own stream inside the timed steps.
-cpu_baseline: the oracle (CPU restatement of FK20Py fk20_multi) on the host cores over a
-bounded sample of the same rows; its outputs also check the GPU rows bit-exactly.
+before timing starts.  Forms measured:
+  F   batched path (fk20_prove_rows): resident window tables, comb MSM, column-batched G1
+      transforms with endomorphism-split twiddles.  `value` is this form.
+  F+graph  the same launch sequence replayed from a captured hipGraph.
+  P2  the reference's six-launch sequence with grid = rows (fk20_poly2h_fft.cu:47-80).
+  P1  the same sequence issued once per row (rows = 1), timed on a bounded sample.
+Every launch of F is timed with HIP events on the stream it runs on; the dominant kernel's
+roofline uses those durations.  cpu_baseline: the oracle (CPU restatement of FK20Py
+fk20_multi) on the host cores over a bounded sample of the same rows; its outputs also
+check the GPU rows bit-exactly.
 """
 import argparse
-import ctypes
 import json
 import os
 import statistics
@@ -27,13 +30,25 @@ import numpy as np  # noqa: E402
 from fk20_amd import lib as L  # noqa: E402
 from fk20_amd import synthetic as S  # noqa: E402
 from fk20_amd import workload as WL  # noqa: E402
+from fk20_amd.prover import LAUNCH_KINDS, Prover  # noqa: E402
 
 R_MOD = S.R_MOD
 METRIC = "FK20 rows proved/sec (N=4096, 512 rows) batched vs plain; p50/p99 latency, bit-exact"
+G1_GEN = np.zeros(18, dtype=np.uint64)
+G1_GEN[:6] = [0xFB3AF00ADB22C6BB, 0x6C55E83FF97A1AEF, 0xA14E3A3F171BAC58,
+              0xC3688C4F9774B905, 0x2695638C4FA9AC0F, 0x17F1D3A73197D794]
+G1_GEN[6:12] = [0x0CAA232946C5E7E1, 0xD03CC744A2888AE4, 0x00DB18CB2C04B3ED,
+                0xFCF5E095D5D00AF6, 0xA09E30ED741D8AE4, 0x08B3F481E3AAA0F1]
+G1_GEN[12] = 1
 
 
 def log(msg):
     print("[bench] " + msg, file=sys.stderr, flush=True)
+
+
+def pct(v, q):
+    v = sorted(v)
+    return v[min(len(v) - 1, int(round(q / 100.0 * (len(v) - 1))))]
 
 
 class Batch:
@@ -54,13 +69,7 @@ class Batch:
         for _ in range(N + 1):
             pw.append(S.int_to_fr(acc))
             acc = acc * self.tau % R_MOD
-        G = np.zeros(18, dtype=np.uint64)
-        G[:6] = [0xFB3AF00ADB22C6BB, 0x6C55E83FF97A1AEF, 0xA14E3A3F171BAC58,
-                 0xC3688C4F9774B905, 0x2695638C4FA9AC0F, 0x17F1D3A73197D794]
-        G[6:12] = [0x0CAA232946C5E7E1, 0xD03CC744A2888AE4, 0x00DB18CB2C04B3ED,
-                   0xFCF5E095D5D00AF6, 0xA09E30ED741D8AE4, 0x08B3F481E3AAA0F1]
-        G[12] = 1
-        gens = L.DeviceBuffer.from_array(np.tile(G, (N + 1, 1)))
+        gens = L.DeviceBuffer.from_array(np.tile(G1_GEN, (N + 1, 1)))
         ks = L.DeviceBuffer.from_array(np.array(pw))
         srs = L.DeviceBuffer((N + 1) * 144)
         L.call("fk20_g1p_op", 3, srs.ptr, gens.ptr, None, ks.ptr, N + 1, None)
@@ -73,77 +82,99 @@ class Batch:
         return xext
 
 
-class Scratch:
-    def __init__(self, rows, n2):
-        self.tc = L.DeviceBuffer(rows * 16 * n2 * 32)
-        self.h = L.DeviceBuffer(rows * n2 * 144)
+# ---------------------------------------------------------------- F: batched path
 
-
-def plain_step(b, sc, stream, ev):
-    """The six-launch sequence (fk20_poly2h_fft.cu:57-77) with an event between stages."""
-    s = stream.handle
-    rows, N, n2 = b.rows, b.N, b.n2
-    ev[0].record(stream)
-    L.call("fk20_poly2toeplitz_coefficients", sc.tc.ptr, b.poly.ptr, rows, N, s)
-    L.call("fr_fft", sc.tc.ptr, sc.tc.ptr, rows * 16, n2, s)
-    ev[1].record(stream)
-    L.call("fk20_msm", sc.h.ptr, sc.tc.ptr, b.xext.ptr, rows, n2, s)
-    ev[2].record(stream)
-    L.call("g1p_ift", sc.h.ptr, sc.h.ptr, rows, n2, s)
-    ev[3].record(stream)
-    L.call("fk20_hext2h", sc.h.ptr, rows, n2, s)
-    L.call("g1p_fft", b.h_fft.ptr, sc.h.ptr, rows, n2, s)
-    ev[4].record(stream)
-
-
-def time_plain(b, steps, warmup):
+def time_fused(b, prover, sid, steps, warmup):
     stream = L.Stream()
-    sc = Scratch(b.rows, b.n2)
-    ev = [L.Event() for _ in range(5)]
+    e0, e1 = L.Event(), L.Event()
+    prover.set_launch_timing(True)
     for _ in range(warmup):
-        plain_step(b, sc, stream, ev)
+        prover.prove_device(sid, b.h_fft, b.poly, b.rows, stream=stream.handle)
     stream.synchronize()
-    step_ms, stage_ms = [], {"ntt": [], "msm": [], "ifft": [], "fft": []}
-    L.synchronize()
-    t0 = time.perf_counter()
+    step_ms, launches = [], {}
     for _ in range(steps):
-        plain_step(b, sc, stream, ev)
-        # events of this step (waits for the step; the next step is issued after)
-        stage_ms["ntt"].append(ev[0].elapsed_ms(ev[1]))
-        stage_ms["msm"].append(ev[1].elapsed_ms(ev[2]))
-        stage_ms["ifft"].append(ev[2].elapsed_ms(ev[3]))
-        stage_ms["fft"].append(ev[3].elapsed_ms(ev[4]))
-        step_ms.append(ev[0].elapsed_ms(ev[4]))
+        e0.record(stream)
+        prover.prove_device(sid, b.h_fft, b.poly, b.rows, stream=stream.handle)
+        e1.record(stream)
+        step_ms.append(e0.elapsed_ms(e1))
+        for kind, ms in prover.launch_times():
+            launches.setdefault(LAUNCH_KINDS[kind], []).append(ms)
+    prover.set_launch_timing(False)
+    # the same sequence replayed from a hipGraph
+    prover.set_graph(True)
+    prover.prove_device(sid, b.h_fft, b.poly, b.rows, stream=stream.handle)  # capture
     stream.synchronize()
-    L.synchronize()
-    wall = time.perf_counter() - t0
-    for e in ev:
-        e.destroy()
+    graph_ms = []
+    for _ in range(steps):
+        e0.record(stream)
+        prover.prove_device(sid, b.h_fft, b.poly, b.rows, stream=stream.handle)
+        e1.record(stream)
+        graph_ms.append(e0.elapsed_ms(e1))
+    prover.set_graph(False)
+    stream.synchronize()
+    e0.destroy()
+    e1.destroy()
     stream.destroy()
-    return step_ms, stage_ms, wall
+    return step_ms, launches, graph_ms
+
+
+# ---------------------------------------------------------------- P2 / P1: plain path
+
+def time_plain(b, steps):
+    stream = L.Stream()
+    tc = L.DeviceBuffer(b.rows * 16 * b.n2 * 32)
+    h = L.DeviceBuffer(b.rows * b.n2 * 144)
+    e0, e1 = L.Event(), L.Event()
+    out = L.DeviceBuffer(b.rows * b.n2 * 144)
+
+    def step():
+        s = stream.handle
+        L.call("fk20_poly2toeplitz_coefficients", tc.ptr, b.poly.ptr, b.rows, b.N, s)
+        L.call("fr_fft", tc.ptr, tc.ptr, b.rows * 16, b.n2, s)
+        L.call("fk20_msm", h.ptr, tc.ptr, b.xext.ptr, b.rows, b.n2, s)
+        L.call("g1p_ift", h.ptr, h.ptr, b.rows, b.n2, s)
+        L.call("fk20_hext2h", h.ptr, b.rows, b.n2, s)
+        L.call("g1p_fft", out.ptr, h.ptr, b.rows, b.n2, s)
+
+    step()
+    stream.synchronize()
+    ms = []
+    for _ in range(steps):
+        e0.record(stream)
+        step()
+        e1.record(stream)
+        ms.append(e0.elapsed_ms(e1))
+    for x in (tc, h, out):
+        x.free()
+    e0.destroy()
+    e1.destroy()
+    stream.destroy()
+    return ms
 
 
 def time_p1(b, sample_rows):
     """Per-row plain launch sequence: fk20_poly2h_fft with rows = 1, row after row."""
     stream = L.Stream()
     e0, e1 = L.Event(), L.Event()
-    row_bytes_in, row_bytes_out = b.N * 32, b.n2 * 144
-    # warm-up row
-    L.call("fk20_poly2h_fft", b.h_fft.ptr, b.poly.ptr, b.xext.ptr, 1, b.N, stream.handle)
+    out = L.DeviceBuffer(sample_rows * b.n2 * 144)
+    L.call("fk20_poly2h_fft", out.ptr, b.poly.ptr, b.xext.ptr, 1, b.N, stream.handle)
     stream.synchronize()
     lat = []
     for r in range(sample_rows):
         e0.record(stream)
-        L.call("fk20_poly2h_fft", b.h_fft.ptr + r * row_bytes_out, b.poly.ptr + r * row_bytes_in,
-               b.xext.ptr, 1, b.N, stream.handle)
+        L.call("fk20_poly2h_fft", out.ptr + r * b.n2 * 144, b.poly.ptr + r * b.N * 32, b.xext.ptr, 1, b.N,
+               stream.handle)
         e1.record(stream)
         lat.append(e0.elapsed_ms(e1))
     stream.synchronize()
+    out.free()
     e0.destroy()
     e1.destroy()
     stream.destroy()
     return lat
 
+
+# ---------------------------------------------------------------- CPU baseline
 
 def cpu_baseline(b, sample_rows, threads):
     """Oracle on host cores over the first sample_rows rows; checks those GPU rows bit-exactly."""
@@ -157,7 +188,7 @@ def cpu_baseline(b, sample_rows, threads):
     got = b.h_fft.download((b.rows, b.n2, 18))[:sample_rows]
     same = np.array_equal(O.to_affine(got.reshape(-1, 18)), O.to_affine(exp.reshape(-1, 18)))
     return {
-        "value": sample_rows / el,
+        "value": round(sample_rows / el, 4),
         "unit": "rows/s",
         "cores": threads,
         "kind": "port",
@@ -169,56 +200,96 @@ def cpu_baseline(b, sample_rows, threads):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--rows", type=int, default=512)
     ap.add_argument("--N", type=int, default=4096)
+    ap.add_argument("--window", type=int, default=8, help="MSM window bits (8, 10, 12)")
     ap.add_argument("--seed", type=int, default=2024)
-    ap.add_argument("--p1-rows", type=int, default=4, help="rows timed for the per-row form P1")
+    ap.add_argument("--plain-steps", type=int, default=2, help="steps of the plain P2 form (0: skip)")
+    ap.add_argument("--p1-rows", type=int, default=4, help="rows timed for the per-row form P1 (0: skip)")
     ap.add_argument("--cpu-rows", type=int, default=16, help="rows in the CPU baseline sample")
     ap.add_argument("--cpu-threads", type=int, default=16)
     ap.add_argument("--no-cpu", action="store_true")
     args = ap.parse_args()
 
     if args.gpus != 1:
-        print("bench.py runs on one GPU only: FK20 rows are independent and this build measures "
-              "the single-GPU path (--gpus %d requested)" % args.gpus, file=sys.stderr)
+        print("bench.py measures one GPU only: FK20 rows are independent, and this build runs "
+              "and reports the single-GPU path (--gpus %d requested)" % args.gpus, file=sys.stderr)
         sys.exit(2)
     if L.device_count() < 1:
         sys.exit("no HIP device visible")
 
     log("generating %d rows, N=%d, setup on device" % (args.rows, args.N))
     b = Batch(args.rows, args.N, args.seed)
-    log("timing plain batched form: %d warmup + %d steps" % (args.warmup, args.steps))
-    step_ms, stage_ms, wall = time_plain(b, args.steps, args.warmup)
+    prover = Prover()
+    t0 = time.perf_counter()
+    sid = prover.load_setup(b.xext, args.N, args.window)
+    setup_s = time.perf_counter() - t0
+    prover.reserve(args.rows, args.N)
+    log("setup tables built in %.2f s (window %d bits)" % (setup_s, args.window))
+
+    log("timing F: %d warmup + %d steps" % (args.warmup, args.steps))
+    step_ms, launches, graph_ms = time_fused(b, prover, sid, args.steps, args.warmup)
     ms = statistics.median(step_ms)
     value = args.rows / (ms / 1e3)
-    log("plain: %.1f ms/step, %.1f rows/s; stages %s" % (
-        ms, value, {k: round(statistics.median(v), 2) for k, v in stage_ms.items()}))
+    log("F: %.2f ms/step p50, %.1f rows/s; graph replay %.2f ms" % (ms, value, statistics.median(graph_ms)))
 
-    log("timing P1 (per-row launches) on %d rows" % args.p1_rows)
-    p1 = time_p1(b, args.p1_rows)
-    p1_rate = 1e3 / statistics.median(p1)
-
-    # roofline of the dominant kernel: the stage with the largest median time
-    stage_med = {k: statistics.median(v) for k, v in stage_ms.items()}
-    dom = max(stage_med, key=stage_med.get)
-    work = WL.stage_mul32(args.N)[dom] * args.rows
-    achieved = work / (stage_med[dom] / 1e3) / 1e12
+    # dominant kernel: the one with the largest total time per step
+    per_step = {k: sum(v) / args.steps for k, v in launches.items()}
+    dom = max(per_step, key=per_step.get)
+    n_launch = int(round(len(launches[dom]) / args.steps))
+    avg_ms = statistics.mean(launches[dom])
+    lg = int(np.log2(args.N // 8))
+    if dom == "k_dit_stage":
+        stages = list(range(lg - 1))
+    elif dom == "k_dif_stage":
+        stages = list(range(1, lg))
+    else:
+        stages = [None]
+    # executed work of this kernel's algorithm per launch (mean over its launches in a step)
+    exe_fp = (sum(WL.executed_launch_fp_muls(args.N, dom, st, args.window) for st in stages)
+              / len(stages) * args.rows)
+    work = exe_fp * WL.FP_MUL32
+    canon = WL.launch_mul32(args.N, dom, args.rows)
+    achieved = work / (avg_ms / 1e3) / 1e12
     peak = WL.PEAK_MUL32_PER_S / 1e12
     roofline = {
         "bound": "valu",
-        "kernel": {"msm": "k_msm_plain", "ifft": "k_g1_fft<9,true>", "fft": "k_g1_fft<9,false>",
-                   "ntt": "k_fr_ntt<9,false>"}[dom],
+        "kernel": dom,
+        "launches_per_step": n_launch,
+        "avg_launch_ms": round(avg_ms, 4),
         "achieved": round(achieved, 4),
         "peak": round(peak, 2),
         "unit": "Tmul32/s",
         "frac": round(achieved / peak, 4),
         "traffic": None,
+        "work": "executed Fp muls of the kernel's algorithm x 300 mul32 (12x32-bit CIOS convention)",
+        "fp_mul_per_s": round(exe_fp / (avg_ms / 1e3), 1),
+        "achieved_canonical": round(canon / (avg_ms / 1e3) / 1e12, 4),
     }
+
+    forms = {"F_rows_per_s": round(value, 3),
+             "F_graph_rows_per_s": round(args.rows / (statistics.median(graph_ms) / 1e3), 3),
+             "F_step_ms_p50": round(ms, 3), "F_step_ms_p99": round(pct(step_ms, 99), 3)}
+    if args.plain_steps:
+        log("timing P2 plain batched grid: %d steps" % args.plain_steps)
+        p2 = time_plain(b, args.plain_steps)
+        forms["P2_rows_per_s"] = round(args.rows / (statistics.median(p2) / 1e3), 3)
+        forms["P2_step_ms_p50"] = round(statistics.median(p2), 3)
+    if args.p1_rows:
+        log("timing P1 per-row launches on %d rows" % args.p1_rows)
+        p1 = time_p1(b, args.p1_rows)
+        forms["P1_rows_per_s"] = round(1e3 / statistics.median(p1), 3)
+        forms["P1_row_latency_ms_p50"] = round(statistics.median(p1), 3)
+        forms["P1_sample_rows"] = args.p1_rows
+        forms["F_over_P1"] = round(value / forms["P1_rows_per_s"], 1)
+    if "P2_rows_per_s" in forms:
+        forms["F_over_P2"] = round(value / forms["P2_rows_per_s"], 2)
 
     cpu, exact = None, None
     if not args.no_cpu:
+        # the F output of the last timed step is in b.h_fft (the graph replays rewrote it)
         log("cpu baseline: oracle over %d rows with %d threads" % (args.cpu_rows, args.cpu_threads))
         cpu, exact = cpu_baseline(b, args.cpu_rows, args.cpu_threads)
 
@@ -235,22 +306,19 @@ def main():
         "vs_baseline": None,
         "dtype": "u32",
         "data": "synthetic: uniform Fr coefficients (PCG64 seed %d), setup from a seeded secret" % args.seed,
-        "config": {"workload": "fk20_batch_plain", "rows": args.rows, "N": args.N, "l": 16,
-                   "fft_size": args.N // 8, "form": "plain batched grid (P2)"},
-        "forms": {
-            "plain_P2_rows_per_s": round(value, 3),
-            "per_row_P1_rows_per_s": round(p1_rate, 3),
-            "per_row_P1_latency_ms_p50": round(statistics.median(p1), 3),
-            "p1_sample_rows": args.p1_rows,
-        },
-        "stage_ms_p50": {k: round(v, 3) for k, v in stage_med.items()},
-        "wall_s": round(wall, 3),
+        "config": {"workload": "fk20_batch", "rows": args.rows, "N": args.N, "l": 16, "fft_size": args.N // 8,
+                   "form": "F batched (fk20_prove_rows)", "msm_window_bits": args.window},
+        "latency_ms": {"step_p50": round(ms, 3), "step_p99": round(pct(step_ms, 99), 3)},
+        "forms": forms,
+        "launch_ms_per_step": {k: round(v, 3) for k, v in sorted(per_step.items(), key=lambda x: -x[1])},
+        "setup_table_build_s": round(setup_s, 3),
         "roofline": roofline,
         "cpu_baseline": cpu,
         "bit_exact_rows_checked": (args.cpu_rows if cpu else 0),
         "bit_exact": exact,
     }
     print(json.dumps(out), flush=True)
+    prover.close()
     if exact is False:
         sys.exit("GPU rows differ from the oracle")
 

@@ -168,8 +168,23 @@ struct fk20_ctx {
         hipStream_t stream;
     } gkey{};
     hipGraphExec_t gexec = nullptr;
+    // per-launch timing: events[0] before the first launch, events[k] after launch k
+    bool timing = false;
+    std::vector<hipEvent_t> events;
+    std::vector<int> kinds;
+    unsigned nmarks = 0;
+    hipStream_t tstream = nullptr;
     std::mutex mu;
 };
+
+namespace {
+void mark_launch(void *arg, int kind) {
+    fk20_ctx *c = static_cast<fk20_ctx *>(arg);
+    if (c->nmarks + 1 >= c->events.size()) return;
+    c->kinds[c->nmarks] = kind;
+    (void)hipEventRecord(c->events[++c->nmarks], c->tstream);
+}
+}  // namespace
 
 extern "C" {
 
@@ -209,6 +224,7 @@ int fk20_ctx_destroy(fk20_ctx_t *c) {
     if (c->prog.ops) (void)hipFree(c->prog.ops);
     if (c->prog.index) (void)hipFree(c->prog.index);
     if (c->gexec) (void)hipGraphExecDestroy(c->gexec);
+    for (hipEvent_t e : c->events) (void)hipEventDestroy(e);
     delete c;
     return FK20_OK;
 }
@@ -301,14 +317,17 @@ int fk20_ctx_reserve(fk20_ctx_t *c, unsigned rows, unsigned N) {
 }
 
 static hipError_t enqueue_rows(fk20_ctx *c, const Setup &st, uint32_t *h_fft, const uint32_t *poly,
-                               unsigned rows, hipStream_t s) {
+                               unsigned rows, hipStream_t s, LaunchHook hook) {
     const unsigned n2 = st.n2;
     uint32_t *tcT = reinterpret_cast<uint32_t *>(c->work);
     uint32_t *D = tcT + (size_t)rows * 16 * n2 * 8;
     hipError_t e = launch_toeplitz_fft_T(tcT, poly, rows, st.N, c->T, s);
+    hook(LK_TOEPLITZ_FFT);
     if (e == hipSuccess) e = launch_msm_comb(D, tcT, st.table, st.inf, rows, n2, st.win, s);
-    if (e == hipSuccess) e = launch_g1_transforms(D, rows, n2, c->prog.ops, c->prog.index, s);
+    hook(LK_MSM);
+    if (e == hipSuccess) e = launch_g1_transforms(D, rows, n2, c->prog.ops, c->prog.index, s, hook);
     if (e == hipSuccess) e = launch_fft_output(h_fft, D, rows, n2, s);
+    hook(LK_OUTPUT);
     return e;
 }
 
@@ -325,7 +344,15 @@ int fk20_prove_rows(fk20_ctx_t *c, int id, fk20_g1p_t *h_fft, const fk20_fr_t *p
     if (e != hipSuccess) return status_of(e);
     uint32_t *out = reinterpret_cast<uint32_t *>(h_fft);
     const uint32_t *in = reinterpret_cast<const uint32_t *>(poly);
-    if (!c->use_graph || s == nullptr) return status_of(enqueue_rows(c, st, out, in, rows, s));
+    if (c->timing) {
+        // direct launches with an event after each one (read back by fk20_ctx_launch_times)
+        c->tstream = s;
+        c->nmarks = 0;
+        e = hipEventRecord(c->events[0], s);
+        if (e != hipSuccess) return status_of(e);
+        return status_of(enqueue_rows(c, st, out, in, rows, s, LaunchHook{mark_launch, c}));
+    }
+    if (!c->use_graph || s == nullptr) return status_of(enqueue_rows(c, st, out, in, rows, s, LaunchHook{nullptr, nullptr}));
     // hipGraph: capture the launch sequence once per (setup, rows, buffers, stream), replay after
     const fk20_ctx::GraphKey key{id, rows, poly, h_fft, s};
     if (!c->gexec || memcmp(&key, &c->gkey, sizeof key) != 0) {
@@ -336,7 +363,7 @@ int fk20_prove_rows(fk20_ctx_t *c, int id, fk20_g1p_t *h_fft, const fk20_fr_t *p
         hipGraph_t g = nullptr;
         e = hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal);
         if (e != hipSuccess) return status_of(e);
-        hipError_t le = enqueue_rows(c, st, out, in, rows, s);
+        hipError_t le = enqueue_rows(c, st, out, in, rows, s, LaunchHook{nullptr, nullptr});
         e = hipStreamEndCapture(s, &g);
         if (le != hipSuccess) e = le;
         if (e == hipSuccess) e = hipGraphInstantiate(&c->gexec, g, nullptr, nullptr, 0);
@@ -345,6 +372,35 @@ int fk20_prove_rows(fk20_ctx_t *c, int id, fk20_g1p_t *h_fft, const fk20_fr_t *p
         c->gkey = key;
     }
     return status_of(hipGraphLaunch(c->gexec, s));
+}
+
+int fk20_ctx_set_launch_timing(fk20_ctx_t *c, int enable) {
+    if (!c) return FK20_EINVAL;
+    std::lock_guard<std::mutex> lk(c->mu);
+    if (enable && c->events.empty()) {
+        c->events.resize(64, nullptr);
+        c->kinds.assign(64, -1);
+        for (hipEvent_t &e : c->events) {
+            hipError_t err = hipEventCreate(&e);
+            if (err != hipSuccess) return status_of(err);
+        }
+    }
+    c->timing = enable != 0;
+    return FK20_OK;
+}
+
+int fk20_ctx_launch_times(fk20_ctx_t *c, float *ms, int *kinds, int max, int *count) {
+    if (!c || !ms || !kinds || !count) return FK20_EINVAL;
+    std::lock_guard<std::mutex> lk(c->mu);
+    const int n = std::min<int>((int)c->nmarks, max);
+    for (int k = 0; k < n; k++) {
+        hipError_t e = hipEventSynchronize(c->events[k + 1]);
+        if (e == hipSuccess) e = hipEventElapsedTime(&ms[k], c->events[k], c->events[k + 1]);
+        if (e != hipSuccess) return status_of(e);
+        kinds[k] = c->kinds[k];
+    }
+    *count = n;
+    return FK20_OK;
 }
 
 int fk20_twiddle_selfcheck(unsigned *total_ops) {

@@ -32,8 +32,18 @@ hipError_t launch_toeplitz_fft_T(uint32_t *tcT, const uint32_t *poly, unsigned r
                                  const DevTables *T, hipStream_t s);
 hipError_t launch_msm_comb(uint32_t *D, const uint32_t *tcT, const uint32_t *table, const uint32_t *inf,
                            unsigned rows, unsigned n2, unsigned win_bits, hipStream_t s);
+// Kernel kinds of the batched path, reported by the per-launch timing of the context.
+enum LaunchKind { LK_TOEPLITZ_FFT = 0, LK_MSM = 1, LK_DIT = 2, LK_MIDDLE = 3, LK_DIF = 4, LK_OUTPUT = 5 };
+// Called after each launch when the context times launches (nullptr otherwise).
+struct LaunchHook {
+    void (*fn)(void *arg, int kind);
+    void *arg;
+    void operator()(int kind) const {
+        if (fn) fn(arg, kind);
+    }
+};
 hipError_t launch_g1_transforms(uint32_t *D, unsigned rows, unsigned n2, const uint32_t *ops,
-                                const uint32_t *index, hipStream_t s);
+                                const uint32_t *index, hipStream_t s, LaunchHook hook);
 hipError_t launch_fft_output(uint32_t *h_fft, const uint32_t *D, unsigned rows, unsigned n2, hipStream_t s);
 
 }  // namespace fk20

@@ -428,16 +428,19 @@ hipError_t launch_msm_comb(uint32_t *D, const uint32_t *tcT, const uint32_t *tab
 }
 
 hipError_t launch_g1_transforms(uint32_t *D, unsigned rows, unsigned n2, const uint32_t *ops,
-                                const uint32_t *index, hipStream_t s) {
+                                const uint32_t *index, hipStream_t s, LaunchHook hook) {
     const unsigned logn2 = n2 == 128 ? 7 : n2 == 256 ? 8 : 9;
     const unsigned nb = n2 / 2;
     for (unsigned st = 0; st + 1 < logn2; st++) {
         hipLaunchKernelGGL(k_dit_stage<TW_WNAF>, dim3(blocks_for(nb, rows)), dim3(64), 0, s, D, rows, st, ops, index);
+        hook(LK_DIT);
     }
     hipLaunchKernelGGL(k_middle<TW_WNAF>, dim3(blocks_for(nb, rows)), dim3(64), 0, s, D, rows, logn2, ops, index);
+    hook(LK_MIDDLE);
     for (unsigned st = 1; st < logn2; st++) {
         hipLaunchKernelGGL(k_dif_stage<TW_WNAF>, dim3(blocks_for(nb, rows)), dim3(64), 0, s, D, rows, logn2, st,
                            ops, index);
+        hook(LK_DIF);
     }
     return hipGetLastError();
 }

@@ -57,6 +57,8 @@ _SIGS = {
     "fk20_ctx_reserve": ([_P, _U, _U], _I),
     "fk20_prove_rows": ([_P, _I, _P, _P, _U, _P], _I),
     "fk20_twiddle_selfcheck": ([ctypes.POINTER(_U)], _I),
+    "fk20_ctx_set_launch_timing": ([_P, _I], _I),
+    "fk20_ctx_launch_times": ([_P, ctypes.POINTER(ctypes.c_float), ctypes.POINTER(_I), _I, ctypes.POINTER(_I)], _I),
     "fk20_status_string": ([_I], ctypes.c_char_p),
     "fk20_device_count": ([ctypes.POINTER(_I)], _I),
     "fk20_set_device": ([_I], _I),

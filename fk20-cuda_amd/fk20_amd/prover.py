@@ -56,6 +56,20 @@ class Prover:
         hb.free()
         return out
 
+    def set_graph(self, enable):
+        call("fk20_ctx_set_graph", self.ctx, int(bool(enable)))
+
+    def set_launch_timing(self, enable):
+        call("fk20_ctx_set_launch_timing", self.ctx, int(bool(enable)))
+
+    def launch_times(self, max_launches=64):
+        """[(kind, ms)] of the last fk20_prove_rows call made with launch timing on."""
+        ms = (ctypes.c_float * max_launches)()
+        kinds = (ctypes.c_int * max_launches)()
+        n = ctypes.c_int()
+        call("fk20_ctx_launch_times", self.ctx, ms, kinds, max_launches, ctypes.byref(n))
+        return [(kinds[i], ms[i]) for i in range(n.value)]
+
     def close(self):
         if self.ctx:
             call("fk20_ctx_destroy", self.ctx)
@@ -76,3 +90,7 @@ def twiddle_selfcheck():
     n = ctypes.c_uint()
     status = lib().fk20_twiddle_selfcheck(ctypes.byref(n))
     return status, n.value
+
+
+LAUNCH_KINDS = {0: "k_toeplitz_fft_T", 1: "k_msm_comb", 2: "k_dit_stage", 3: "k_middle", 4: "k_dif_stage",
+                5: "k_fft_output"}

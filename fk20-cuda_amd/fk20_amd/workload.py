@@ -55,3 +55,119 @@ def stage_mul32(N):
 # HBM bytes a row must move at minimum: polynomial in, h_fft out (projective)
 def row_hbm_bytes(N):
     return N * 32 + (N // 8) * 144
+
+
+def launch_mul32(N, kind, rows):
+    """Canonical work of one launch of the batched path over `rows` rows.
+
+    k_msm_comb: the MSM; k_dit_stage / k_dif_stage: one radix-2 stage of (n2/2) butterflies;
+    k_middle: the last IFFT and first FFT stages (two stages).  The IFFT's n2 scaling
+    multiplications (folded into the setup tables here) are not attributed to any launch.
+    """
+    n2 = N // 8
+    stage = (n2 // 2) * BF * FP_MUL32
+    return rows * {
+        "k_msm_comb": 16 * n2 * (SM + ADD) * FP_MUL32,
+        "k_dit_stage": stage,
+        "k_dif_stage": stage,
+        "k_middle": 2 * stage,
+        "k_toeplitz_fft_T": 16 * (n2 // 2) * int(math.log2(n2)) * FR_MUL32,
+        "k_fft_output": 0,
+    }[kind]
+
+
+# ---------------------------------------------------------------- executed-work model
+#
+# Fp multiplications the batched path's algorithms actually perform (same 300-mul32
+# convention), for the roofline of its kernels:
+#   twiddle w^e: GLV split w^e = k2*lambda + k1, wNAF-4 digits of k1 and k2; table
+#     P, 3P, 5P, 7P (1 dbl + 3 add); one dbl per digit position after the first, one add
+#     per non-zero digit after the first, one Fp mul per phi-digit
+#   butterfly: twiddle (if e != 0) + 2 complete additions
+#   MSM: one mixed addition (11 Fp mul) per non-zero signed window digit, 16 terms per column
+
+R_MOD = 0x73EDA753299D7D483339D80809A1D80553BDA402FFFE5BFEFFFFFFFF00000001
+GLV_LAMBDA = 0xAC45A4010001A40200000000FFFFFFFF
+MIXED_ADD = 11
+WNAF_W = 4
+
+
+def _wnaf(k, w):
+    d = []
+    while k:
+        if k & 1:
+            x = k & ((1 << w) - 1)
+            if x >= 1 << (w - 1):
+                x -= 1 << w
+            k -= x
+        else:
+            x = 0
+        d.append(x)
+        k >>= 1
+    return d
+
+
+def twiddle_program(e_idx, inverse=False):
+    """(doublings, additions, phi_muls, ops) of the program for w^e (w of order 512)."""
+    w = pow(5, (R_MOD - 1) // 512, R_MOD)
+    k = pow(w, (512 - e_idx) % 512 if inverse else e_idx, R_MOD)
+    k2, k1 = divmod(k, GLV_LAMBDA)
+    d1, d2 = _wnaf(k1, WNAF_W), _wnaf(k2, WNAF_W)
+    L = max(len(d1), len(d2))
+    dbl = adds = phi = ops = 0
+    started = False
+    for t in range(L - 1, -1, -1):
+        if started:
+            dbl += 1
+        for part, d in ((0, d1), (1, d2)):
+            x = d[t] if t < len(d) else 0
+            if x:
+                ops += 1
+                phi += part
+                if started:
+                    adds += 1
+                started = True
+    return dbl, adds, phi, ops + 1
+
+
+def twiddle_fp_muls(e_idx, inverse=False):
+    dbl, adds, phi, _ = twiddle_program(e_idx, inverse)
+    table = DBL + 3 * ADD
+    return table + dbl * DBL + adds * ADD + phi
+
+
+def executed_launch_fp_muls(N, kind, stage=None, window=8):
+    """Executed Fp muls of one launch of the batched path, per row."""
+    n2 = N // 8
+    lg = int(math.log2(n2))
+    if kind == "k_msm_comb":
+        # ceil(256/w) signed windows per term; zero digits (probability ~2^-w) ignored
+        return 16 * n2 * ((256 + window - 1) // window) * MIXED_ADD
+    if kind in ("k_dit_stage", "k_dif_stage"):
+        total = 0
+        for b in range(n2 // 2):
+            if kind == "k_dit_stage":
+                j = b & ((1 << stage) - 1)
+                idx = j << (8 - stage)
+            else:
+                lh = lg - stage - 1
+                j = b & ((1 << lh) - 1)
+                idx = j << (8 - lh)
+            total += 2 * ADD + (twiddle_fp_muls(idx, inverse=(kind == "k_dit_stage")) if idx else 0)
+        return total
+    if kind == "k_middle":
+        total = 0
+        for j in range(n2 // 2):
+            idx = j << (9 - lg)
+            total += ADD + (twiddle_fp_muls(idx, True) + twiddle_fp_muls(idx, False) if idx else 0)
+        return total
+    raise KeyError(kind)
+
+
+def executed_row_fp_muls(N, window=8):
+    lg = int(math.log2(N // 8))
+    t = executed_launch_fp_muls(N, "k_msm_comb", window=window)
+    t += sum(executed_launch_fp_muls(N, "k_dit_stage", s) for s in range(lg - 1))
+    t += executed_launch_fp_muls(N, "k_middle")
+    t += sum(executed_launch_fp_muls(N, "k_dif_stage", s) for s in range(1, lg))
+    return t

@@ -137,6 +137,12 @@ int fk20_ctx_reserve(fk20_ctx_t *ctx, unsigned rows, unsigned N);
  * MSM, column-batched G1 IFFT / zero / FFT with endomorphism-split twiddles. */
 int fk20_prove_rows(fk20_ctx_t *ctx, int setup_id, fk20_g1p_t *h_fft, const fk20_fr_t *polynomial,
                     unsigned rows, fk20_stream_t stream);
+/* Per-launch timing of fk20_prove_rows (direct launches, graph replay off while enabled):
+ * an event is recorded after every kernel; fk20_ctx_launch_times waits for the last call's
+ * events and returns per-launch milliseconds and kinds (0 Toeplitz+NTT, 1 MSM, 2 IFFT stage,
+ * 3 middle stage, 4 FFT stage, 5 output). */
+int fk20_ctx_set_launch_timing(fk20_ctx_t *ctx, int enable);
+int fk20_ctx_launch_times(fk20_ctx_t *ctx, float *ms, int *kinds, int max, int *count);
 /* Host-only check of the twiddle programs (GLV split + wNAF of every w^+-i, i < 512,
  * evaluated over Fr); total_ops receives the program size.  Needs no device. */
 int fk20_twiddle_selfcheck(unsigned *total_ops);

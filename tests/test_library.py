@@ -75,3 +75,11 @@ def test_setup_table_bytes():
     assert prover.setup_table_bytes(4096, 8) == 16 * 512 * 32 * 128 * 96
     assert prover.setup_table_bytes(1024, 10) == 16 * 128 * 26 * 512 * 96
     assert prover.setup_table_bytes(4096, 9) == 0
+
+
+def test_executed_work_model_matches_library_programs():
+    """The Python work model rebuilds the same twiddle programs the library runs."""
+    from fk20_amd import prover, workload
+    _, nops = prover.twiddle_selfcheck()
+    model = sum(workload.twiddle_program(e, inv)[3] for inv in (False, True) for e in range(512))
+    assert model == nops
