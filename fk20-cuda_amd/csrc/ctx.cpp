@@ -160,13 +160,8 @@ struct fk20_ctx {
     void *work = nullptr;
     size_t work_bytes = 0;
     bool use_graph = false;
-    struct GraphKey {
-        int setup;
-        unsigned rows;
-        const void *poly;
-        void *out;
-        hipStream_t stream;
-    } gkey{};
+    std::vector<fk20_batch_item_t> gkey;  // items of the captured graph
+    hipStream_t gstream = nullptr;
     hipGraphExec_t gexec = nullptr;
     // per-launch timing: events[0] before the first launch, events[k] after launch k
     bool timing = false;
@@ -285,9 +280,10 @@ int fk20_setup_unload(fk20_ctx_t *c, int id) {
     (void)hipFree(s.table);
     (void)hipFree(s.inf);
     s = Setup();
-    if (c->gexec && c->gkey.setup == id) {
+    if (c->gexec) {
         (void)hipGraphExecDestroy(c->gexec);
         c->gexec = nullptr;
+        c->gkey.clear();
     }
     return FK20_OK;
 }
@@ -316,46 +312,77 @@ int fk20_ctx_reserve(fk20_ctx_t *c, unsigned rows, unsigned N) {
     return status_of(grow_work(c, fk20_rows_workspace_bytes(rows, N)));
 }
 
-static hipError_t enqueue_rows(fk20_ctx *c, const Setup &st, uint32_t *h_fft, const uint32_t *poly,
-                               unsigned rows, hipStream_t s, LaunchHook hook) {
-    const unsigned n2 = st.n2;
-    uint32_t *tcT = reinterpret_cast<uint32_t *>(c->work);
-    uint32_t *D = tcT + (size_t)rows * 16 * n2 * 8;
-    hipError_t e = launch_toeplitz_fft_T(tcT, poly, rows, st.N, c->T, s);
-    hook(LK_TOEPLITZ_FFT);
-    if (e == hipSuccess) e = launch_msm_comb(D, tcT, st.table, st.inf, rows, n2, st.win, s);
-    hook(LK_MSM);
-    if (e == hipSuccess) e = launch_g1_transforms(D, rows, n2, c->prog.ops, c->prog.index, s, hook);
-    if (e == hipSuccess) e = launch_fft_output(h_fft, D, rows, n2, s);
-    hook(LK_OUTPUT);
-    return e;
+static size_t items_workspace_bytes(const fk20_ctx *c, const fk20_batch_item_t *items, unsigned n) {
+    size_t total = 0;
+    for (unsigned k = 0; k < n; k++) total += fk20_rows_workspace_bytes(items[k].rows, c->setups[items[k].setup_id].N);
+    return total;
 }
 
-int fk20_prove_rows(fk20_ctx_t *c, int id, fk20_g1p_t *h_fft, const fk20_fr_t *poly, unsigned rows,
-                    fk20_stream_t stream) {
-    if (!c) return FK20_EINVAL;
+static bool same_items(const std::vector<fk20_batch_item_t> &a, const fk20_batch_item_t *b, unsigned n) {
+    if (a.size() != n) return false;
+    for (unsigned k = 0; k < n; k++)
+        if (a[k].setup_id != b[k].setup_id || a[k].rows != b[k].rows || a[k].polynomial != b[k].polynomial ||
+            a[k].h_fft != b[k].h_fft)
+            return false;
+    return true;
+}
+
+static hipError_t enqueue_items(fk20_ctx *c, const fk20_batch_item_t *items, unsigned n, hipStream_t s,
+                                LaunchHook hook) {
+    Launch L;
+    memset(&L, 0, sizeof L);
+    L.nitems = n;
+    char *w = reinterpret_cast<char *>(c->work);
+    for (unsigned k = 0; k < n; k++) {
+        const Setup &st = c->setups[items[k].setup_id];
+        Item &it = L.item[k];
+        const size_t rows = items[k].rows;
+        it.poly = reinterpret_cast<const uint32_t *>(items[k].polynomial);
+        it.h_fft = reinterpret_cast<uint32_t *>(items[k].h_fft);
+        it.tcT = reinterpret_cast<uint32_t *>(w);
+        w += rows * 16 * st.n2 * 32;
+        it.D = reinterpret_cast<uint32_t *>(w);
+        w += rows * st.n2 * 144;
+        it.table = st.table;
+        it.inf = st.inf;
+        it.rows = items[k].rows;
+        it.N = st.N;
+        it.logn2 = st.n2 == 128 ? 7 : st.n2 == 256 ? 8 : 9;
+        it.win = st.win;
+        it.stage = -1;
+    }
+    return launch_batch(L, c->T, c->prog.ops, c->prog.index, s, hook);
+}
+
+int fk20_prove_batch(fk20_ctx_t *c, const fk20_batch_item_t *items, unsigned nitems, fk20_stream_t stream) {
+    if (!c || (nitems && !items) || nitems > MAX_ITEMS) return FK20_EINVAL;
     std::lock_guard<std::mutex> lk(c->mu);
-    if (id < 0 || id >= (int)c->setups.size() || !c->setups[id].live) return FK20_EINVAL;
-    if (!rows) return FK20_OK;
-    if (!h_fft || !poly) return FK20_EINVAL;
-    const Setup &st = c->setups[id];
+    // drop empty items; validate the rest
+    std::vector<fk20_batch_item_t> v;
+    for (unsigned k = 0; k < nitems; k++) {
+        const fk20_batch_item_t &it = items[k];
+        if (it.setup_id < 0 || it.setup_id >= (int)c->setups.size() || !c->setups[it.setup_id].live)
+            return FK20_EINVAL;
+        if (!it.rows) continue;
+        if (!it.polynomial || !it.h_fft) return FK20_EINVAL;
+        v.push_back(it);
+    }
+    if (v.empty()) return FK20_OK;
+    const unsigned n = (unsigned)v.size();
     hipStream_t s = reinterpret_cast<hipStream_t>(stream);
-    hipError_t e = grow_work(c, fk20_rows_workspace_bytes(rows, st.N));
+    hipError_t e = grow_work(c, items_workspace_bytes(c, v.data(), n));
     if (e != hipSuccess) return status_of(e);
-    uint32_t *out = reinterpret_cast<uint32_t *>(h_fft);
-    const uint32_t *in = reinterpret_cast<const uint32_t *>(poly);
     if (c->timing) {
         // direct launches with an event after each one (read back by fk20_ctx_launch_times)
         c->tstream = s;
         c->nmarks = 0;
         e = hipEventRecord(c->events[0], s);
         if (e != hipSuccess) return status_of(e);
-        return status_of(enqueue_rows(c, st, out, in, rows, s, LaunchHook{mark_launch, c}));
+        return status_of(enqueue_items(c, v.data(), n, s, LaunchHook{mark_launch, c}));
     }
-    if (!c->use_graph || s == nullptr) return status_of(enqueue_rows(c, st, out, in, rows, s, LaunchHook{nullptr, nullptr}));
-    // hipGraph: capture the launch sequence once per (setup, rows, buffers, stream), replay after
-    const fk20_ctx::GraphKey key{id, rows, poly, h_fft, s};
-    if (!c->gexec || memcmp(&key, &c->gkey, sizeof key) != 0) {
+    if (!c->use_graph || s == nullptr) return status_of(enqueue_items(c, v.data(), n, s, LaunchHook{nullptr, nullptr}));
+    // hipGraph: capture the launch sequence once per (items, stream), replay after
+    if (!c->gexec || c->gstream != s || !same_items(c->gkey, v.data(), n)) {
         if (c->gexec) {
             (void)hipGraphExecDestroy(c->gexec);
             c->gexec = nullptr;
@@ -363,15 +390,27 @@ int fk20_prove_rows(fk20_ctx_t *c, int id, fk20_g1p_t *h_fft, const fk20_fr_t *p
         hipGraph_t g = nullptr;
         e = hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal);
         if (e != hipSuccess) return status_of(e);
-        hipError_t le = enqueue_rows(c, st, out, in, rows, s, LaunchHook{nullptr, nullptr});
+        hipError_t le = enqueue_items(c, v.data(), n, s, LaunchHook{nullptr, nullptr});
         e = hipStreamEndCapture(s, &g);
         if (le != hipSuccess) e = le;
         if (e == hipSuccess) e = hipGraphInstantiate(&c->gexec, g, nullptr, nullptr, 0);
         if (g) (void)hipGraphDestroy(g);
         if (e != hipSuccess) return status_of(e);
-        c->gkey = key;
+        c->gkey = v;
+        c->gstream = s;
     }
     return status_of(hipGraphLaunch(c->gexec, s));
+}
+
+int fk20_prove_rows(fk20_ctx_t *c, int id, fk20_g1p_t *h_fft, const fk20_fr_t *poly, unsigned rows,
+                    fk20_stream_t stream) {
+    if (!c) return FK20_EINVAL;
+    fk20_batch_item_t it;
+    it.setup_id = id;
+    it.rows = rows;
+    it.polynomial = poly;
+    it.h_fft = h_fft;
+    return fk20_prove_batch(c, &it, 1, stream);
 }
 
 int fk20_ctx_set_launch_timing(fk20_ctx_t *c, int enable) {

@@ -28,13 +28,27 @@ hipError_t launch_scale_bases(uint32_t *xs, uint32_t *inf, const uint32_t *xext,
                               const uint32_t *scale, hipStream_t s);
 hipError_t launch_table_build(uint32_t *table, uint32_t *zt, const uint32_t *xs, const uint32_t *inf,
                               unsigned base0, unsigned nb, unsigned win_bits, unsigned nwin, hipStream_t s);
-hipError_t launch_toeplitz_fft_T(uint32_t *tcT, const uint32_t *poly, unsigned rows, unsigned N,
-                                 const DevTables *T, hipStream_t s);
-hipError_t launch_msm_comb(uint32_t *D, const uint32_t *tcT, const uint32_t *table, const uint32_t *inf,
-                           unsigned rows, unsigned n2, unsigned win_bits, hipStream_t s);
+// One group of rows sharing a setup (hence N) and buffers; the unit of a batched launch.
+static constexpr unsigned MAX_ITEMS = 16;
+struct Item {
+    const uint32_t *poly;   // [rows][N] fr_t
+    uint32_t *h_fft;        // [rows][n2] g1p_t
+    uint32_t *tcT;          // workspace [n2][16][rows] fr_t
+    uint32_t *D;            // workspace [n2][rows] packed Montgomery points
+    const uint32_t *table;  // setup window tables
+    const uint32_t *inf;    // setup infinity flags
+    uint32_t rows, N, logn2, win;
+    int32_t stage;          // set per launch by launch_batch
+};
+struct Launch {
+    Item item[MAX_ITEMS];
+    uint32_t blk_end[MAX_ITEMS];  // per-item block prefix sums of the current launch
+    uint32_t nitems;
+};
+
 // Kernel kinds of the batched path, reported by the per-launch timing of the context.
 enum LaunchKind { LK_TOEPLITZ_FFT = 0, LK_MSM = 1, LK_DIT = 2, LK_MIDDLE = 3, LK_DIF = 4, LK_OUTPUT = 5 };
-// Called after each launch when the context times launches (nullptr otherwise).
+// Called after each launch when the context times launches (fn == nullptr otherwise).
 struct LaunchHook {
     void (*fn)(void *arg, int kind);
     void *arg;
@@ -42,8 +56,10 @@ struct LaunchHook {
         if (fn) fn(arg, kind);
     }
 };
-hipError_t launch_g1_transforms(uint32_t *D, unsigned rows, unsigned n2, const uint32_t *ops,
-                                const uint32_t *index, hipStream_t s, LaunchHook hook);
-hipError_t launch_fft_output(uint32_t *h_fft, const uint32_t *D, unsigned rows, unsigned n2, hipStream_t s);
+
+// The whole batched path for the items of `L`: Toeplitz + NTT, MSM, 2*Lmax - 1 transform
+// stages, output -- one launch per stage covering every item.
+hipError_t launch_batch(const Launch &L, const DevTables *T, const uint32_t *ops, const uint32_t *index,
+                        hipStream_t s, LaunchHook hook);
 
 }  // namespace fk20

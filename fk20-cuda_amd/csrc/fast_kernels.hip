@@ -158,6 +158,22 @@ __global__ void __launch_bounds__(64) k_table_build(uint32_t *table, uint32_t *z
     }
 }
 
+// ---------------------------------------------------------------- multi-item decoding
+//
+// Every launch below covers a list of items (row groups, each with its own setup, N and
+// buffers).  A block finds its item from the per-item block prefix sums of the launch.
+
+__device__ __forceinline__ unsigned find_item(const Launch &L, unsigned blk, unsigned &local) {
+    unsigned k = 0, start = 0;
+#pragma unroll 1
+    while (k + 1 < L.nitems && blk >= L.blk_end[k]) {
+        start = L.blk_end[k];
+        k++;
+    }
+    local = blk - start;
+    return k;
+}
+
 // ---------------------------------------------------------------- Toeplitz + NTT (transposed)
 
 __device__ __forceinline__ int toeplitz_src_T(unsigned i, unsigned j, unsigned N) {
@@ -167,73 +183,81 @@ __device__ __forceinline__ int toeplitz_src_T(unsigned i, unsigned j, unsigned N
     return (int)(31 - i + 16 * (j - (k + 2)));
 }
 
-template <unsigned LOGN>
-__global__ void __launch_bounds__(256) k_toeplitz_fft_T(uint32_t *tcT, const uint32_t *poly, unsigned rows,
-                                                        unsigned N, const DevTables *T) {
-    constexpr unsigned n = 1u << LOGN, half = n / 2;
-    __shared__ fs lds[n];
+// One (row, i) NTT of size n2 <= 512 per workgroup of 256 threads (n2/2 of them active).
+__global__ void __launch_bounds__(256) k_toeplitz_fft_multi(Launch L, const DevTables *T) {
+    __shared__ fs lds[512];
+    unsigned local;
+    const Item &it = L.item[find_item(L, blockIdx.x, local)];
+    const unsigned logn = it.logn2, n = 1u << logn, half = n / 2, N = it.N;
     const unsigned t = threadIdx.x;
-    const unsigned i = blockIdx.x % 16;
-    const size_t row = blockIdx.x / 16;
-    poly += row * N * FR_WORDS;
-    for (unsigned e = t; e < n; e += half) {
-        const int src = toeplitz_src_T(i, e, N);
-        fs a;
-        fs_zero(a);
-        if (src >= 0) {
-            uint32_t w[8];
-            ld_words(w, poly + (size_t)src * FR_WORDS, 2);
+    const unsigned i = local % 16;
+    const size_t row = local / 16;
+    const uint32_t *poly = it.poly + row * N * FR_WORDS;
+    if (t < half) {
+        for (unsigned e = t; e < n; e += half) {
+            const int src = toeplitz_src_T(i, e, N);
+            fs a;
+            fs_zero(a);
+            if (src >= 0) {
+                uint32_t w[8];
+                ld_words(w, poly + (size_t)src * FR_WORDS, 2);
 #pragma unroll
-            for (int q = 0; q < 8; q++) a.l[q] = w[q];
-            fs_to_mont(a, a);
+                for (int q = 0; q < 8; q++) a.l[q] = w[q];
+                fs_to_mont(a, a);
+            }
+            lds[bitrev(e, logn)] = a;
         }
-        lds[bitrev(e, LOGN)] = a;
     }
     __syncthreads();
-    for (unsigned s = 0; s < LOGN; s++) {
-        const unsigned h = 1u << s, j = t & (h - 1);
-        const unsigned l = ((t >> s) << (s + 1)) + j, r = l + h;
-        fs x = lds[r], y = lds[l], w;
-        fs_set(w, T->tw_mont[j << (8 - s)]);
-        fs_mul(x, x, w);
-        fs_add(lds[l], y, x);
-        fs_sub(lds[r], y, x);
+    for (unsigned s = 0; s < logn; s++) {
+        if (t < half) {
+            const unsigned h = 1u << s, j = t & (h - 1);
+            const unsigned l = ((t >> s) << (s + 1)) + j, r = l + h;
+            fs x = lds[r], y = lds[l], w;
+            fs_set(w, T->tw_mont[j << (8 - s)]);
+            fs_mul(x, x, w);
+            fs_add(lds[l], y, x);
+            fs_sub(lds[r], y, x);
+        }
         __syncthreads();
     }
-    for (unsigned e = t; e < n; e += half) {
-        fs a = lds[e];
-        fs_from_mont(a, a);
-        st_words(tcT + (((size_t)e * 16 + i) * rows + row) * FR_WORDS, a.l, 2);
+    if (t < half) {
+        for (unsigned e = t; e < n; e += half) {
+            fs a = lds[e];
+            fs_from_mont(a, a);
+            st_words(it.tcT + (((size_t)e * 16 + i) * it.rows + row) * FR_WORDS, a.l, 2);
+        }
     }
 }
 
-// ---------------------------------------------------------------- comb MSM
+// ---------------------------------------------------------------- window-table MSM
 
-template <unsigned WIN>
-__global__ void __launch_bounds__(64) k_msm_comb(uint32_t *D, const uint32_t *tcT, const uint32_t *table,
-                                                 const uint32_t *inf, unsigned rows, unsigned n2,
-                                                 unsigned logn2) {
-    constexpr unsigned NWIN = (256 + WIN - 1) / WIN, H = 1u << (WIN - 1), MASK = (1u << WIN) - 1;
-    const unsigned rg = (rows + 63) / 64;
-    const unsigned j = blockIdx.x / rg;
-    const unsigned row = (blockIdx.x % rg) * 64 + threadIdx.x;
+// hext_fft[j] for 64 rows of one column j: sum over 16 terms of ceil(256/w) signed-window
+// digits, one mixed addition each.  Output to D[bitrev(j)][row].
+__global__ void __launch_bounds__(64) k_msm_multi(Launch L) {
+    unsigned local;
+    const Item &it = L.item[find_item(L, blockIdx.x, local)];
+    const unsigned rows = it.rows, n2 = 1u << it.logn2, rg = (rows + 63) / 64;
+    const unsigned WIN = it.win, NWIN = (256 + WIN - 1) / WIN, H = 1u << (WIN - 1), MASK = (1u << WIN) - 1;
+    const unsigned j = local / rg;
+    const unsigned row = (local % rg) * 64 + threadIdx.x;
     if (row >= rows) return;
     g1p acc;
     g1_inf(acc);
 #pragma unroll 1
     for (unsigned i = 0; i < 16; i++) {
         const unsigned b = i * n2 + j;
-        if (inf[b]) continue;
+        if (it.inf[b]) continue;
         uint32_t k[8];
-        ld_words(k, tcT + (((size_t)j * 16 + i) * rows + row) * FR_WORDS, 2);
-        const uint32_t *tb = table + (size_t)b * NWIN * H * G1A_WORDS;
+        ld_words(k, it.tcT + (((size_t)j * 16 + i) * rows + row) * FR_WORDS, 2);
+        const uint32_t *tb = it.table + (size_t)b * NWIN * H * G1A_WORDS;
         uint32_t carry = 0;
 #pragma unroll 1
         for (unsigned win = 0; win < NWIN; win++) {
             const uint32_t bits = (k[0] & MASK) + carry;
             carry = bits > H ? 1u : 0u;
             const int d = (int)bits - (int)(carry << WIN);
-            // shift the scalar right by WIN bits
+            // shift the scalar right by WIN bits (WIN < 32)
 #pragma unroll
             for (int q = 0; q < 7; q++) k[q] = (k[q] >> WIN) | (k[q + 1] << (32 - WIN));
             k[7] >>= WIN;
@@ -248,7 +272,7 @@ __global__ void __launch_bounds__(64) k_msm_comb(uint32_t *D, const uint32_t *tc
             }
         }
     }
-    pt_store(D + ((size_t)bitrev(j, logn2) * rows + row) * 36, acc);
+    pt_store(it.D + ((size_t)bitrev(j, it.logn2) * rows + row) * 36, acc);
 }
 
 // ---------------------------------------------------------------- twiddle programs
@@ -299,14 +323,24 @@ __device__ __forceinline__ void tw_mul(g1p &x, const uint32_t *ops) {
     x = acc;
 }
 
-// IFFT decimation-in-time stage s over D[pos][row] (bit-reversed input), twiddles w^-idx.
+// ---------------------------------------------------------------- G1 transform stages
+//
+// Per item with L = log2(n2), stage schedule over the 2*Lmax - 1 launches of a batch:
+//   IFFT DIT stage s (0 <= s <= L-2)  at launch Lmax-1-(L-1)+s   (right-aligned)
+//   middle                            at launch Lmax-1            (shared)
+//   FFT DIF stage s (1 <= s <= L-1)   at launch Lmax-1+s          (left-aligned)
+// so every item of a launch runs the same kind; it.stage holds s (or -1: idle).
+
+// IFFT decimation in time (bit-reversed input), twiddles w^-idx before the butterfly.
 template <int W>
-__global__ void __launch_bounds__(64) k_dit_stage(uint32_t *D, unsigned rows, unsigned s,
-                                                  const uint32_t *ops, const uint32_t *index) {
-    const unsigned rg = (rows + 63) / 64;
-    const unsigned b = blockIdx.x / rg;
-    const unsigned row = (blockIdx.x % rg) * 64 + threadIdx.x;
+__global__ void __launch_bounds__(64) k_dit_multi(Launch L, const uint32_t *ops, const uint32_t *index) {
+    unsigned local;
+    const Item &it = L.item[find_item(L, blockIdx.x, local)];
+    const unsigned rows = it.rows, rg = (rows + 63) / 64, s = (unsigned)it.stage;
+    const unsigned b = local / rg;
+    const unsigned row = (local % rg) * 64 + threadIdx.x;
     if (row >= rows) return;
+    uint32_t *D = it.D;
     const unsigned j = b & ((1u << s) - 1);
     const unsigned l = ((b >> s) << (s + 1)) + j, r = l + (1u << s);
     const unsigned idx = j << (8 - s);
@@ -324,12 +358,14 @@ __global__ void __launch_bounds__(64) k_dit_stage(uint32_t *D, unsigned rows, un
 // Last IFFT stage, upper half dropped (h = first half), then the first FFT DIF stage whose
 // upper input half is zero:  x = E_j + w^-idx O_j;  D[j] = x;  D[j + n2/2] = w^idx x.
 template <int W>
-__global__ void __launch_bounds__(64) k_middle(uint32_t *D, unsigned rows, unsigned logn2,
-                                               const uint32_t *ops, const uint32_t *index) {
-    const unsigned rg = (rows + 63) / 64;
-    const unsigned j = blockIdx.x / rg;
-    const unsigned row = (blockIdx.x % rg) * 64 + threadIdx.x;
+__global__ void __launch_bounds__(64) k_middle_multi(Launch L, const uint32_t *ops, const uint32_t *index) {
+    unsigned local;
+    const Item &it = L.item[find_item(L, blockIdx.x, local)];
+    const unsigned rows = it.rows, rg = (rows + 63) / 64, logn2 = it.logn2;
+    const unsigned j = local / rg;
+    const unsigned row = (local % rg) * 64 + threadIdx.x;
     if (row >= rows) return;
+    uint32_t *D = it.D;
     const unsigned hn = 1u << (logn2 - 1);
     const unsigned idx = j << (9 - logn2);
     g1p o, e;
@@ -342,15 +378,17 @@ __global__ void __launch_bounds__(64) k_middle(uint32_t *D, unsigned rows, unsig
     pt_store(D + ((size_t)(j + hn) * rows + row) * 36, e);
 }
 
-// FFT decimation-in-frequency stage s >= 1 (span n2 >> (s+1)), twiddles w^idx after the
+// FFT decimation in frequency stage s >= 1 (span n2 >> (s+1)), twiddles w^idx after the
 // subtraction.
 template <int W>
-__global__ void __launch_bounds__(64) k_dif_stage(uint32_t *D, unsigned rows, unsigned logn2, unsigned s,
-                                                  const uint32_t *ops, const uint32_t *index) {
-    const unsigned rg = (rows + 63) / 64;
-    const unsigned b = blockIdx.x / rg;
-    const unsigned row = (blockIdx.x % rg) * 64 + threadIdx.x;
+__global__ void __launch_bounds__(64) k_dif_multi(Launch L, const uint32_t *ops, const uint32_t *index) {
+    unsigned local;
+    const Item &it = L.item[find_item(L, blockIdx.x, local)];
+    const unsigned rows = it.rows, rg = (rows + 63) / 64, logn2 = it.logn2, s = (unsigned)it.stage;
+    const unsigned b = local / rg;
+    const unsigned row = (local % rg) * 64 + threadIdx.x;
     if (row >= rows) return;
+    uint32_t *D = it.D;
     const unsigned lh = logn2 - s - 1;  // log2 of the span
     const unsigned h = 1u << lh;
     const unsigned j = b & (h - 1);
@@ -368,24 +406,21 @@ __global__ void __launch_bounds__(64) k_dif_stage(uint32_t *D, unsigned rows, un
 }
 
 // D[pos][row] (bit-reversed) -> h_fft[row][bitrev(pos)], canonical standard projective.
-__global__ void __launch_bounds__(64) k_fft_output(uint32_t *h_fft, const uint32_t *D, unsigned rows,
-                                                   unsigned logn2) {
-    const unsigned rg = (rows + 63) / 64;
-    const unsigned pos = blockIdx.x / rg;
-    const unsigned row = (blockIdx.x % rg) * 64 + threadIdx.x;
+__global__ void __launch_bounds__(64) k_output_multi(Launch L) {
+    unsigned local;
+    const Item &it = L.item[find_item(L, blockIdx.x, local)];
+    const unsigned rows = it.rows, rg = (rows + 63) / 64, logn2 = it.logn2;
+    const unsigned pos = local / rg;
+    const unsigned row = (local % rg) * 64 + threadIdx.x;
     if (row >= rows) return;
     g1p p;
-    pt_load(p, D + ((size_t)pos * rows + row) * 36);
+    pt_load(p, it.D + ((size_t)pos * rows + row) * 36);
     uint32_t w[36];
     g1_store_std(w, p);
-    st_words(h_fft + ((size_t)row * (1u << logn2) + bitrev(pos, logn2)) * G1P_WORDS, w, 9);
+    st_words(it.h_fft + ((size_t)row * (1u << logn2) + bitrev(pos, logn2)) * G1P_WORDS, w, 9);
 }
 
 // ---------------------------------------------------------------- launchers
-
-static inline unsigned blocks_for(unsigned per_row_units, unsigned rows) {
-    return per_row_units * ((rows + 63) / 64);
-}
 
 hipError_t launch_scale_bases(uint32_t *xs, uint32_t *inf, const uint32_t *xext, size_t count,
                               const uint32_t *scale, hipStream_t s) {
@@ -402,52 +437,65 @@ hipError_t launch_table_build(uint32_t *table, uint32_t *zt, const uint32_t *xs,
     return hipGetLastError();
 }
 
-hipError_t launch_toeplitz_fft_T(uint32_t *tcT, const uint32_t *poly, unsigned rows, unsigned N,
-                                 const DevTables *T, hipStream_t s) {
-    const unsigned blocks = rows * 16;
-    switch (N) {
-    case 1024: hipLaunchKernelGGL(k_toeplitz_fft_T<7>, dim3(blocks), dim3(64), 0, s, tcT, poly, rows, N, T); break;
-    case 2048: hipLaunchKernelGGL(k_toeplitz_fft_T<8>, dim3(blocks), dim3(128), 0, s, tcT, poly, rows, N, T); break;
-    case 4096: hipLaunchKernelGGL(k_toeplitz_fft_T<9>, dim3(blocks), dim3(256), 0, s, tcT, poly, rows, N, T); break;
-    default: return hipErrorInvalidValue;
+// Fills L.blk_end from per-item block counts; returns the total.
+static unsigned set_blocks(Launch &L, const unsigned *per_item) {
+    unsigned total = 0;
+    for (unsigned k = 0; k < L.nitems; k++) {
+        total += per_item[k];
+        L.blk_end[k] = total;
     }
-    return hipGetLastError();
+    return total;
 }
 
-hipError_t launch_msm_comb(uint32_t *D, const uint32_t *tcT, const uint32_t *table, const uint32_t *inf,
-                           unsigned rows, unsigned n2, unsigned win_bits, hipStream_t s) {
-    const unsigned logn2 = n2 == 128 ? 7 : n2 == 256 ? 8 : 9;
-    const dim3 grid(blocks_for(n2, rows));
-    switch (win_bits) {
-    case 8: hipLaunchKernelGGL(k_msm_comb<8>, grid, dim3(64), 0, s, D, tcT, table, inf, rows, n2, logn2); break;
-    case 10: hipLaunchKernelGGL(k_msm_comb<10>, grid, dim3(64), 0, s, D, tcT, table, inf, rows, n2, logn2); break;
-    case 12: hipLaunchKernelGGL(k_msm_comb<12>, grid, dim3(64), 0, s, D, tcT, table, inf, rows, n2, logn2); break;
-    default: return hipErrorInvalidValue;
-    }
-    return hipGetLastError();
-}
-
-hipError_t launch_g1_transforms(uint32_t *D, unsigned rows, unsigned n2, const uint32_t *ops,
-                                const uint32_t *index, hipStream_t s, LaunchHook hook) {
-    const unsigned logn2 = n2 == 128 ? 7 : n2 == 256 ? 8 : 9;
-    const unsigned nb = n2 / 2;
-    for (unsigned st = 0; st + 1 < logn2; st++) {
-        hipLaunchKernelGGL(k_dit_stage<TW_WNAF>, dim3(blocks_for(nb, rows)), dim3(64), 0, s, D, rows, st, ops, index);
+hipError_t launch_batch(const Launch &base, const DevTables *T, const uint32_t *ops, const uint32_t *index,
+                        hipStream_t s, LaunchHook hook) {
+    Launch L = base;
+    unsigned per[MAX_ITEMS], lmax = 0;
+    for (unsigned k = 0; k < L.nitems; k++) lmax = std::max(lmax, L.item[k].logn2);
+    // Toeplitz + NTT
+    for (unsigned k = 0; k < L.nitems; k++) per[k] = L.item[k].rows * 16;
+    unsigned total = set_blocks(L, per);
+    if (total) hipLaunchKernelGGL(k_toeplitz_fft_multi, dim3(total), dim3(256), 0, s, L, T);
+    hook(LK_TOEPLITZ_FFT);
+    // MSM
+    for (unsigned k = 0; k < L.nitems; k++) per[k] = (1u << L.item[k].logn2) * ((L.item[k].rows + 63) / 64);
+    total = set_blocks(L, per);
+    if (total) hipLaunchKernelGGL(k_msm_multi, dim3(total), dim3(64), 0, s, L);
+    hook(LK_MSM);
+    // IFFT DIT stages, right-aligned to the middle launch
+    for (unsigned t = 0; t + 1 < lmax; t++) {
+        for (unsigned k = 0; k < L.nitems; k++) {
+            const int st = (int)t - (int)(lmax - L.item[k].logn2);
+            L.item[k].stage = st;
+            per[k] = st >= 0 ? (1u << (L.item[k].logn2 - 1)) * ((L.item[k].rows + 63) / 64) : 0;
+        }
+        total = set_blocks(L, per);
+        if (total) hipLaunchKernelGGL(k_dit_multi<TW_WNAF>, dim3(total), dim3(64), 0, s, L, ops, index);
         hook(LK_DIT);
     }
-    hipLaunchKernelGGL(k_middle<TW_WNAF>, dim3(blocks_for(nb, rows)), dim3(64), 0, s, D, rows, logn2, ops, index);
+    for (unsigned k = 0; k < L.nitems; k++) {
+        L.item[k].stage = 0;
+        per[k] = (1u << (L.item[k].logn2 - 1)) * ((L.item[k].rows + 63) / 64);
+    }
+    total = set_blocks(L, per);
+    if (total) hipLaunchKernelGGL(k_middle_multi<TW_WNAF>, dim3(total), dim3(64), 0, s, L, ops, index);
     hook(LK_MIDDLE);
-    for (unsigned st = 1; st < logn2; st++) {
-        hipLaunchKernelGGL(k_dif_stage<TW_WNAF>, dim3(blocks_for(nb, rows)), dim3(64), 0, s, D, rows, logn2, st,
-                           ops, index);
+    // FFT DIF stages 1.., left-aligned after the middle launch
+    for (unsigned t = 1; t < lmax; t++) {
+        for (unsigned k = 0; k < L.nitems; k++) {
+            const int st = t < L.item[k].logn2 ? (int)t : -1;
+            L.item[k].stage = st;
+            per[k] = st >= 0 ? (1u << (L.item[k].logn2 - 1)) * ((L.item[k].rows + 63) / 64) : 0;
+        }
+        total = set_blocks(L, per);
+        if (total) hipLaunchKernelGGL(k_dif_multi<TW_WNAF>, dim3(total), dim3(64), 0, s, L, ops, index);
         hook(LK_DIF);
     }
-    return hipGetLastError();
-}
-
-hipError_t launch_fft_output(uint32_t *h_fft, const uint32_t *D, unsigned rows, unsigned n2, hipStream_t s) {
-    const unsigned logn2 = n2 == 128 ? 7 : n2 == 256 ? 8 : 9;
-    hipLaunchKernelGGL(k_fft_output, dim3(blocks_for(n2, rows)), dim3(64), 0, s, h_fft, D, rows, logn2);
+    // output
+    for (unsigned k = 0; k < L.nitems; k++) per[k] = (1u << L.item[k].logn2) * ((L.item[k].rows + 63) / 64);
+    total = set_blocks(L, per);
+    if (total) hipLaunchKernelGGL(k_output_multi, dim3(total), dim3(64), 0, s, L);
+    hook(LK_OUTPUT);
     return hipGetLastError();
 }
 

@@ -56,6 +56,7 @@ _SIGS = {
     "fk20_rows_workspace_bytes": ([_U, _U], _SZ),
     "fk20_ctx_reserve": ([_P, _U, _U], _I),
     "fk20_prove_rows": ([_P, _I, _P, _P, _U, _P], _I),
+    "fk20_prove_batch": ([_P, _P, _U, _P], _I),
     "fk20_twiddle_selfcheck": ([ctypes.POINTER(_U)], _I),
     "fk20_ctx_set_launch_timing": ([_P, _I], _I),
     "fk20_ctx_launch_times": ([_P, ctypes.POINTER(ctypes.c_float), ctypes.POINTER(_I), _I, ctypes.POINTER(_I)], _I),

@@ -7,6 +7,15 @@ import numpy as np
 from .lib import DeviceBuffer, call, lib, synchronize
 
 
+class BatchItem(ctypes.Structure):
+    """fk20_batch_item_t of include/fk20.h."""
+    _fields_ = [("setup_id", ctypes.c_int), ("rows", ctypes.c_uint),
+                ("polynomial", ctypes.c_void_p), ("h_fft", ctypes.c_void_p)]
+
+
+MAX_ITEMS = 16
+
+
 class Prover:
     """One libfk20 context on the current device; setups stay resident until closed."""
 
@@ -41,6 +50,36 @@ class Prover:
         hp = h_fft.ptr if isinstance(h_fft, DeviceBuffer) else h_fft
         pp = poly.ptr if isinstance(poly, DeviceBuffer) else poly
         call("fk20_prove_rows", self.ctx, sid, hp, pp, rows, stream)
+
+    def prove_batch_device(self, items, stream=None):
+        """items: [(setup_id, rows, poly_ptr_or_buffer, h_fft_ptr_or_buffer)] in one batched pass."""
+        arr = (BatchItem * max(1, len(items)))()
+        for k, (sid, rows, poly, out) in enumerate(items):
+            arr[k].setup_id = sid
+            arr[k].rows = rows
+            arr[k].polynomial = poly.ptr if isinstance(poly, DeviceBuffer) else poly
+            arr[k].h_fft = out.ptr if isinstance(out, DeviceBuffer) else out
+        call("fk20_prove_batch", self.ctx, arr, len(items), stream)
+
+    def prove_batch(self, groups):
+        """groups: [(setup_id, host polys (rows, N, 4))] -> [host h_fft (rows, N/8, 18)]."""
+        bufs, items = [], []
+        for sid, poly in groups:
+            poly = np.ascontiguousarray(poly, dtype=np.uint64)
+            rows, N = poly.shape[0], poly.shape[1]
+            assert self.setups[sid] == N
+            pb = DeviceBuffer.from_array(poly)
+            hb = DeviceBuffer(rows * (N // 8) * 144)
+            bufs.append((pb, hb, rows, N))
+            items.append((sid, rows, pb, hb))
+        self.prove_batch_device(items)
+        synchronize()
+        outs = []
+        for pb, hb, rows, N in bufs:
+            outs.append(hb.download((rows, N // 8, 18)))
+            pb.free()
+            hb.free()
+        return outs
 
     def prove(self, sid, poly):
         """Host polynomials (rows, N, 4) -> host h_fft (rows, N/8, 18)."""

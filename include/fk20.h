@@ -137,6 +137,17 @@ int fk20_ctx_reserve(fk20_ctx_t *ctx, unsigned rows, unsigned N);
  * MSM, column-batched G1 IFFT / zero / FFT with endomorphism-split twiddles. */
 int fk20_prove_rows(fk20_ctx_t *ctx, int setup_id, fk20_g1p_t *h_fft, const fk20_fr_t *polynomial,
                     unsigned rows, fk20_stream_t stream);
+/* Several row groups in one batched pass (each launch of the path covers every group):
+ * groups may use different setups and polynomial lengths N (1024/2048/4096), e.g. the
+ * uneven batches of a serving loop.  At most 16 groups per call. */
+typedef struct {
+    int setup_id;
+    unsigned rows;
+    const fk20_fr_t *polynomial;  /* [rows][N of the setup] */
+    fk20_g1p_t *h_fft;            /* [rows][N/8] */
+} fk20_batch_item_t;
+int fk20_prove_batch(fk20_ctx_t *ctx, const fk20_batch_item_t *items, unsigned nitems,
+                     fk20_stream_t stream);
 /* Per-launch timing of fk20_prove_rows (direct launches, graph replay off while enabled):
  * an event is recorded after every kernel; fk20_ctx_launch_times waits for the last call's
  * events and returns per-launch milliseconds and kinds (0 Toeplitz+NTT, 1 MSM, 2 IFFT stage,

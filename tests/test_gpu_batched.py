@@ -102,3 +102,24 @@ def test_graph_replay_tracks_input_contents():
         st.destroy()
     finally:
         p.close()
+
+
+def test_uneven_batch_one_pass(prover):
+    """Rows of N = 1024 / 2048 / 4096 with three setups in one fk20_prove_batch call."""
+    groups, taus = [], {}
+    for N, rows, seed in ((1024, 70, 301), (2048, 33, 302), (4096, 20, 303)):
+        tau = S.random_secret(seed)
+        xext = O.xext_fft_from_tau(tau, N)
+        sid = prover.load_setup(xext, N, 8)
+        poly = S.random_polys(rows, N, seed + 10)
+        groups.append((sid, poly))
+        taus[sid] = (tau, xext)
+    outs = prover.prove_batch(groups)
+    for (sid, poly), got in zip(groups, outs):
+        tau, xext = taus[sid]
+        alone = prover.prove(sid, poly)
+        assert np.array_equal(aff(got), aff(alone))
+        for r in (0, poly.shape[0] - 1):
+            assert np.array_equal(aff(got[r]), aff(shadow_points(poly[r], tau)))
+    for sid in list(taus):
+        prover.unload_setup(sid)
