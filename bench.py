@@ -51,6 +51,24 @@ def pct(v, q):
     return v[min(len(v) - 1, int(round(q / 100.0 * (len(v) - 1))))]
 
 
+def device_setup(N, tau):
+    """SRS [tau^i]G on the GPU, then xext_fft by fk20_setup2xext_fft (FK20Py toeplitz_part1)."""
+    pw, acc = [], 1
+    for _ in range(N + 1):
+        pw.append(S.int_to_fr(acc))
+        acc = acc * tau % R_MOD
+    gens = L.DeviceBuffer.from_array(np.tile(G1_GEN, (N + 1, 1)))
+    ks = L.DeviceBuffer.from_array(np.array(pw))
+    srs = L.DeviceBuffer((N + 1) * 144)
+    L.call("fk20_g1p_op", 3, srs.ptr, gens.ptr, None, ks.ptr, N + 1, None)
+    xext = L.DeviceBuffer(16 * (N // 8) * 144)
+    L.call("fk20_setup2xext_fft", xext.ptr, srs.ptr, N, None)
+    L.synchronize()
+    for b in (gens, ks, srs):
+        b.free()
+    return xext
+
+
 class Batch:
     """Device-resident inputs, setup and outputs of one batch."""
 
@@ -60,26 +78,8 @@ class Batch:
         self.poly = L.DeviceBuffer.from_array(self.poly_host)
         self.h_fft = L.DeviceBuffer(rows * self.n2 * 144)
         self.tau = S.random_secret(seed + 1)
-        self.xext = self._setup()
-
-    def _setup(self):
-        """SRS [tau^i]G on the GPU, then xext_fft by fk20_setup2xext_fft (FK20Py toeplitz_part1)."""
-        N = self.N
-        pw, acc = [], 1
-        for _ in range(N + 1):
-            pw.append(S.int_to_fr(acc))
-            acc = acc * self.tau % R_MOD
-        gens = L.DeviceBuffer.from_array(np.tile(G1_GEN, (N + 1, 1)))
-        ks = L.DeviceBuffer.from_array(np.array(pw))
-        srs = L.DeviceBuffer((N + 1) * 144)
-        L.call("fk20_g1p_op", 3, srs.ptr, gens.ptr, None, ks.ptr, N + 1, None)
-        xext = L.DeviceBuffer(16 * self.n2 * 144)
-        L.call("fk20_setup2xext_fft", xext.ptr, srs.ptr, N, None)
-        L.synchronize()
-        self.xext_host = xext.download((16, self.n2, 18))
-        for b in (gens, ks, srs):
-            b.free()
-        return xext
+        self.xext = device_setup(N, self.tau)
+        self.xext_host = self.xext.download((16, self.n2, 18))
 
 
 # ---------------------------------------------------------------- F: batched path
@@ -174,6 +174,130 @@ def time_p1(b, sample_rows):
     return lat
 
 
+# ---------------------------------------------------------------- C4: uneven batch
+
+def time_uneven(prover, rows_each, window, steps, seed):
+    """512 rows each of N = 1024 / 2048 / 4096 (three setups) in one fk20_prove_batch pass."""
+    items, bufs = [], []
+    for k, N in enumerate((1024, 2048, 4096)):
+        xext = device_setup(N, S.random_secret(seed + 100 + k))
+        sid = prover.load_setup(xext, N, window)
+        xext.free()
+        pb = L.DeviceBuffer.from_array(S.random_polys(rows_each, N, seed + 200 + k))
+        hb = L.DeviceBuffer(rows_each * (N // 8) * 144)
+        bufs += [pb, hb]
+        items.append((sid, rows_each, pb, hb))
+    stream = L.Stream()
+    e0, e1 = L.Event(), L.Event()
+    prover.prove_batch_device(items, stream=stream.handle)
+    stream.synchronize()
+    ms = []
+    for _ in range(steps):
+        e0.record(stream)
+        prover.prove_batch_device(items, stream=stream.handle)
+        e1.record(stream)
+        ms.append(e0.elapsed_ms(e1))
+    for sid, _, _, _ in items:
+        prover.unload_setup(sid)
+    for x in bufs:
+        x.free()
+    e0.destroy()
+    e1.destroy()
+    stream.destroy()
+    return ms
+
+
+# ---------------------------------------------------------------- C5: serving loop
+
+def bench_serve(args):
+    """Poisson arrivals of one-row blobs, N = 1024/2048/4096 weighted 1:2:5, offered load swept."""
+    import threading
+
+    from fk20_amd.lib import PinnedArray
+    from fk20_amd.server import Server
+
+    prover = Prover()
+    sids, polys, outs = {}, {}, {}
+    pool = 2048
+    for k, N in enumerate((1024, 2048, 4096)):
+        xext = device_setup(N, S.random_secret(args.seed + 100 + k))
+        sids[N] = prover.load_setup(xext, N, args.window)
+        xext.free()
+        polys[N] = PinnedArray((64, N, 4))
+        polys[N].array[:] = S.random_polys(64, N, args.seed + 300 + k)
+        outs[N] = PinnedArray((pool, N // 8, 18))
+    sizes = np.array([1024] * 1 + [2048] * 2 + [4096] * 5)
+    rates = [float(r) for r in args.rates.split(",")]
+    levels = []
+    for rate in rates:
+        srv = Server(prover, budget_bytes=args.budget_gb << 30, slots=args.slots, max_batch_rows=args.max_batch_rows)
+        rng = np.random.default_rng(args.seed + int(rate))
+        n = max(64, int(rate * args.duration))
+        gaps = rng.exponential(1.0 / rate, size=n)
+        which = sizes[rng.integers(0, len(sizes), size=n)]
+        ids = [None] * n
+        lat = [None] * n
+        done_at = [None] * n
+        submitted = threading.Semaphore(0)
+
+        def waiter():
+            for i in range(n):
+                submitted.acquire()
+                lat[i] = srv.wait(ids[i])
+                done_at[i] = time.perf_counter()
+
+        wt = threading.Thread(target=waiter)
+        wt.start()
+        t0 = time.perf_counter()
+        t_next = t0
+        for i in range(n):
+            t_next += gaps[i]
+            d = t_next - time.perf_counter()
+            if d > 0:
+                time.sleep(d)
+            N = int(which[i])
+            slot = i % pool
+            ids[i] = srv.submit(sids[N], polys[N].ptr + (i % 64) * N * 32, 1,
+                                outs[N].ptr + slot * (N // 8) * 144)
+            submitted.release()
+        wt.join()
+        t_end = max(done_at)
+        st = srv.stats()
+        srv.close()
+        lat_ms = np.array(lat) * 1e3
+        levels.append({
+            "offered_rows_per_s": rate,
+            "sustained_rows_per_s": round(n / (t_end - t0), 1),
+            "p50_ms": round(float(np.percentile(lat_ms, 50)), 3),
+            "p99_ms": round(float(np.percentile(lat_ms, 99)), 3),
+            "requests": n,
+            "batches": st["batches"],
+            "max_inflight_rows": st["max_inflight_rows"],
+        })
+        log("serve %.0f rows/s offered: sustained %.0f, p50 %.2f ms, p99 %.2f ms, %d batches"
+            % (rate, levels[-1]["sustained_rows_per_s"], levels[-1]["p50_ms"], levels[-1]["p99_ms"], st["batches"]))
+        if levels[-1]["sustained_rows_per_s"] < 0.8 * rate:
+            break  # past saturation
+    for a in list(polys.values()) + list(outs.values()):
+        a.free()
+    prover.close()
+    best = max(levels, key=lambda x: x["sustained_rows_per_s"])
+    out = {
+        "metric": "FK20 serving: sustained rows/s and p50/p99 latency under Poisson load",
+        "value": best["sustained_rows_per_s"],
+        "unit": "rows/s",
+        "n_gpus": 1,
+        "higher_is_better": True,
+        "dtype": "u32",
+        "data": "synthetic one-row blobs, N=1024/2048/4096 weighted 1:2:5, Poisson arrivals",
+        "config": {"workload": "fk20_serving", "budget_bytes": args.budget_gb << 30, "slots": args.slots,
+                   "max_batch_rows": args.max_batch_rows, "msm_window_bits": args.window,
+                   "duration_s_per_level": args.duration},
+        "levels": levels,
+    }
+    print(json.dumps(out), flush=True)
+
+
 # ---------------------------------------------------------------- CPU baseline
 
 def cpu_baseline(b, sample_rows, threads):
@@ -211,6 +335,15 @@ def main():
     ap.add_argument("--cpu-rows", type=int, default=16, help="rows in the CPU baseline sample")
     ap.add_argument("--cpu-threads", type=int, default=16)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--uneven-rows", type=int, default=512,
+                    help="rows per size in the uneven N=1024/2048/4096 batch (0: skip)")
+    ap.add_argument("--workload", default="batch", choices=["batch", "serve"])
+    ap.add_argument("--rates", default="1000,2000,4000,6000,8000,10000,14000",
+                    help="serve: offered loads in rows/s")
+    ap.add_argument("--duration", type=float, default=2.0, help="serve: seconds per load level")
+    ap.add_argument("--budget-gb", type=int, default=8, help="serve: device budget for in-flight rows")
+    ap.add_argument("--slots", type=int, default=2, help="serve: batches in flight")
+    ap.add_argument("--max-batch-rows", type=int, default=1024)
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -219,6 +352,10 @@ def main():
         sys.exit(2)
     if L.device_count() < 1:
         sys.exit("no HIP device visible")
+
+    if args.workload == "serve":
+        bench_serve(args)
+        return
 
     log("generating %d rows, N=%d, setup on device" % (args.rows, args.N))
     b = Batch(args.rows, args.N, args.seed)
@@ -286,6 +423,12 @@ def main():
         forms["F_over_P1"] = round(value / forms["P1_rows_per_s"], 1)
     if "P2_rows_per_s" in forms:
         forms["F_over_P2"] = round(value / forms["P2_rows_per_s"], 2)
+    uneven = None
+    if args.uneven_rows:
+        log("timing uneven batch: %d rows each of N=1024/2048/4096 in one pass" % args.uneven_rows)
+        um = time_uneven(prover, args.uneven_rows, args.window, max(2, args.steps // 2), args.seed)
+        uneven = {"rows": 3 * args.uneven_rows, "ms_p50": round(statistics.median(um), 3),
+                  "rows_per_s": round(3 * args.uneven_rows / (statistics.median(um) / 1e3), 1)}
 
     cpu, exact = None, None
     if not args.no_cpu:
@@ -310,6 +453,7 @@ def main():
                    "form": "F batched (fk20_prove_rows)", "msm_window_bits": args.window},
         "latency_ms": {"step_p50": round(ms, 3), "step_p99": round(pct(step_ms, 99), 3)},
         "forms": forms,
+        "uneven_batch": uneven,
         "launch_ms_per_step": {k: round(v, 3) for k, v in sorted(per_step.items(), key=lambda x: -x[1])},
         "setup_table_build_s": round(setup_s, 3),
         "roofline": roofline,

@@ -328,11 +328,11 @@ static bool same_items(const std::vector<fk20_batch_item_t> &a, const fk20_batch
 }
 
 static hipError_t enqueue_items(fk20_ctx *c, const fk20_batch_item_t *items, unsigned n, hipStream_t s,
-                                LaunchHook hook) {
+                                LaunchHook hook, void *workspace = nullptr) {
     Launch L;
     memset(&L, 0, sizeof L);
     L.nitems = n;
-    char *w = reinterpret_cast<char *>(c->work);
+    char *w = reinterpret_cast<char *>(workspace ? workspace : c->work);
     for (unsigned k = 0; k < n; k++) {
         const Setup &st = c->setups[items[k].setup_id];
         Item &it = L.item[k];
@@ -400,6 +400,34 @@ int fk20_prove_batch(fk20_ctx_t *c, const fk20_batch_item_t *items, unsigned nit
         c->gstream = s;
     }
     return status_of(hipGraphLaunch(c->gexec, s));
+}
+
+int fk20_prove_batch_ws(fk20_ctx_t *c, const fk20_batch_item_t *items, unsigned nitems, void *workspace,
+                        size_t workspace_bytes, fk20_stream_t stream) {
+    if (!c || (nitems && !items) || nitems > MAX_ITEMS || !workspace) return FK20_EINVAL;
+    std::lock_guard<std::mutex> lk(c->mu);
+    std::vector<fk20_batch_item_t> v;
+    for (unsigned k = 0; k < nitems; k++) {
+        const fk20_batch_item_t &it = items[k];
+        if (it.setup_id < 0 || it.setup_id >= (int)c->setups.size() || !c->setups[it.setup_id].live)
+            return FK20_EINVAL;
+        if (!it.rows) continue;
+        if (!it.polynomial || !it.h_fft) return FK20_EINVAL;
+        v.push_back(it);
+    }
+    if (v.empty()) return FK20_OK;
+    if (items_workspace_bytes(c, v.data(), (unsigned)v.size()) > workspace_bytes) return FK20_ENOMEM;
+    return status_of(enqueue_items(c, v.data(), (unsigned)v.size(), reinterpret_cast<hipStream_t>(stream),
+                                   LaunchHook{nullptr, nullptr}, workspace));
+}
+
+int fk20_setup_info(fk20_ctx_t *c, int id, unsigned *N, unsigned *window) {
+    if (!c) return FK20_EINVAL;
+    std::lock_guard<std::mutex> lk(c->mu);
+    if (id < 0 || id >= (int)c->setups.size() || !c->setups[id].live) return FK20_EINVAL;
+    if (N) *N = c->setups[id].N;
+    if (window) *window = c->setups[id].win;
+    return FK20_OK;
 }
 
 int fk20_prove_rows(fk20_ctx_t *c, int id, fk20_g1p_t *h_fft, const fk20_fr_t *poly, unsigned rows,

@@ -57,6 +57,16 @@ _SIGS = {
     "fk20_ctx_reserve": ([_P, _U, _U], _I),
     "fk20_prove_rows": ([_P, _I, _P, _P, _U, _P], _I),
     "fk20_prove_batch": ([_P, _P, _U, _P], _I),
+    "fk20_prove_batch_ws": ([_P, _P, _U, _P, _SZ, _P], _I),
+    "fk20_setup_info": ([_P, _I, ctypes.POINTER(_U), ctypes.POINTER(_U)], _I),
+    "fk20_server_create": ([ctypes.POINTER(_P), _P, _SZ, _U, _U], _I),
+    "fk20_server_submit": ([_P, _I, _P, _U, _P, ctypes.POINTER(ctypes.c_uint64)], _I),
+    "fk20_server_wait": ([_P, ctypes.c_uint64, ctypes.POINTER(ctypes.c_double)], _I),
+    "fk20_server_stats": ([_P, ctypes.POINTER(ctypes.c_uint64), ctypes.POINTER(ctypes.c_uint64),
+                           ctypes.POINTER(ctypes.c_uint64)], _I),
+    "fk20_server_destroy": ([_P], _I),
+    "fk20_host_alloc": ([ctypes.POINTER(_P), _SZ], _I),
+    "fk20_host_free": ([_P], _I),
     "fk20_twiddle_selfcheck": ([ctypes.POINTER(_U)], _I),
     "fk20_ctx_set_launch_timing": ([_P, _I], _I),
     "fk20_ctx_launch_times": ([_P, ctypes.POINTER(ctypes.c_float), ctypes.POINTER(_I), _I, ctypes.POINTER(_I)], _I),
@@ -191,3 +201,22 @@ def device_count():
 
 def synchronize():
     call("fk20_synchronize")
+
+
+class PinnedArray:
+    """A numpy view of pinned host memory (hipHostMalloc through the library)."""
+
+    def __init__(self, shape, dtype=np.uint64):
+        self.shape, self.dtype = tuple(shape), np.dtype(dtype)
+        nbytes = int(np.prod(self.shape)) * self.dtype.itemsize
+        p = ctypes.c_void_p()
+        call("fk20_host_alloc", ctypes.byref(p), max(nbytes, 1))
+        self.ptr = p.value
+        buf = (ctypes.c_char * max(nbytes, 1)).from_address(self.ptr)
+        self.array = np.frombuffer(buf, dtype=self.dtype, count=int(np.prod(self.shape))).reshape(self.shape)
+
+    def free(self):
+        if self.ptr:
+            self.array = None
+            call("fk20_host_free", self.ptr)
+            self.ptr = None

@@ -148,6 +148,41 @@ typedef struct {
 } fk20_batch_item_t;
 int fk20_prove_batch(fk20_ctx_t *ctx, const fk20_batch_item_t *items, unsigned nitems,
                      fk20_stream_t stream);
+/* fk20_prove_batch with a caller-provided device workspace of at least the sum of
+ * fk20_rows_workspace_bytes over the items (no allocation; used by the serving loop). */
+int fk20_prove_batch_ws(fk20_ctx_t *ctx, const fk20_batch_item_t *items, unsigned nitems,
+                        void *workspace, size_t workspace_bytes, fk20_stream_t stream);
+/* N and window of a loaded setup. */
+int fk20_setup_info(fk20_ctx_t *ctx, int setup_id, unsigned *N, unsigned *window);
+
+/* ------------------------------------------------------------------ serving loop
+ *
+ * Continuous batching over a context (no reference counterpart: the reference proves
+ * fixed batches).  Requests are host polynomial blobs (rows x N of one loaded setup) with a
+ * host output buffer; best in pinned memory (fk20_host_alloc) so copies are asynchronous.
+ * A worker thread keeps up to `nslots` batches in flight, one per stream; a free slot takes
+ * pending requests in arrival order while they fit its share (budget_bytes / nslots) of the
+ * device-memory budget for in-flight data -- per row the toeplitz_fft (16*N/8 Fr) and hext
+ * (N/8 G1) workspace plus the polynomial and h_fft -- and at most max_batch_rows rows and
+ * 16 setups, packs them per setup into one batched pass, and copies results back.
+ * Requests arriving while batches run are admitted as slots free up; results are the same
+ * as proving each request alone. */
+typedef struct fk20_server fk20_server_t;
+int fk20_server_create(fk20_server_t **server, fk20_ctx_t *ctx, size_t budget_bytes, unsigned nslots,
+                       unsigned max_batch_rows);
+/* FK20_ENOMEM if the request alone exceeds one slot's share of the budget. */
+int fk20_server_submit(fk20_server_t *server, int setup_id, const fk20_fr_t *host_polynomial,
+                       unsigned rows, fk20_g1p_t *host_h_fft, uint64_t *request_id);
+/* Blocks until the request is done; latency = completion - submission, seconds. */
+int fk20_server_wait(fk20_server_t *server, uint64_t request_id, double *latency_s);
+/* Batches launched, rows completed, peak rows in flight; returns the first error seen. */
+int fk20_server_stats(fk20_server_t *server, uint64_t *batches, uint64_t *rows_done,
+                      uint64_t *max_inflight_rows);
+/* Drains pending and in-flight requests, then frees the slots. */
+int fk20_server_destroy(fk20_server_t *server);
+int fk20_host_alloc(void **ptr, size_t bytes);  /* pinned host memory */
+int fk20_host_free(void *ptr);
+
 /* Per-launch timing of fk20_prove_rows (direct launches, graph replay off while enabled):
  * an event is recorded after every kernel; fk20_ctx_launch_times waits for the last call's
  * events and returns per-launch milliseconds and kinds (0 Toeplitz+NTT, 1 MSM, 2 IFFT stage,
