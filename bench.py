@@ -328,7 +328,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--rows", type=int, default=512)
     ap.add_argument("--N", type=int, default=4096)
-    ap.add_argument("--window", type=int, default=8, help="MSM window bits (8, 10, 12)")
+    ap.add_argument("--window", type=int, default=0,
+                    help="MSM window bits (8, 10, 12); 0: the autotuned choice (tuned.json) or 8")
     ap.add_argument("--seed", type=int, default=2024)
     ap.add_argument("--plain-steps", type=int, default=2, help="steps of the plain P2 form (0: skip)")
     ap.add_argument("--p1-rows", type=int, default=4, help="rows timed for the per-row form P1 (0: skip)")
@@ -353,6 +354,9 @@ def main():
     if L.device_count() < 1:
         sys.exit("no HIP device visible")
 
+    if not args.window:
+        from fk20_amd.autotune import tuned_window
+        args.window = tuned_window(args.N)
     if args.workload == "serve":
         bench_serve(args)
         return

@@ -19,7 +19,10 @@ static constexpr uint32_t TWOP_NEG = 1u << 12;
 static constexpr uint32_t TWOP_END = 1u << 31;
 
 // wNAF width used by the G1 transform kernels (odd multiples P, 3P, 5P, 7P)
-static constexpr int TW_WNAF = 4;
+#ifndef FK20_TW_WNAF
+#define FK20_TW_WNAF 4
+#endif
+static constexpr int TW_WNAF = FK20_TW_WNAF;
 // direction rows of the program index: index[dir * 512 + idx] = offset of w^(+-idx)'s program
 static constexpr int TW_FWD = 0;
 static constexpr int TW_INV = 1;

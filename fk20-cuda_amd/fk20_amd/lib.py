@@ -9,7 +9,8 @@ import os
 import numpy as np
 
 PKG_DIR = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
-LIBPATH = os.path.join(PKG_DIR, "libfk20.so")
+# FK20_LIBRARY selects another build of the library (e.g. a variant built for comparison)
+LIBPATH = os.environ.get("FK20_LIBRARY", os.path.join(PKG_DIR, "libfk20.so"))
 
 FK20_OK = 0
 STATUS = {0: "ok", -1: "invalid argument", -2: "out of device memory", -3: "HIP runtime error",
