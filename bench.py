@@ -177,11 +177,14 @@ def time_p1(b, sample_rows):
 # ---------------------------------------------------------------- C4: uneven batch
 
 def time_uneven(prover, rows_each, window, steps, seed):
-    """512 rows each of N = 1024 / 2048 / 4096 (three setups) in one fk20_prove_batch pass."""
+    """512 rows each of N = 1024 / 2048 / 4096 (three setups) in one fk20_prove_batch pass;
+    each setup uses its autotuned window (tuned.json) or `window`."""
+    from fk20_amd.autotune import tuned_window
+
     items, bufs = [], []
     for k, N in enumerate((1024, 2048, 4096)):
         xext = device_setup(N, S.random_secret(seed + 100 + k))
-        sid = prover.load_setup(xext, N, window)
+        sid = prover.load_setup(xext, N, tuned_window(N, window))
         xext.free()
         pb = L.DeviceBuffer.from_array(S.random_polys(rows_each, N, seed + 200 + k))
         hb = L.DeviceBuffer(rows_each * (N // 8) * 144)
