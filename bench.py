@@ -92,6 +92,7 @@ def time_fused(b, prover, sid, steps, warmup):
         prover.prove_device(sid, b.h_fft, b.poly, b.rows, stream=stream.handle)
     stream.synchronize()
     step_ms, launches = [], {}
+    L.synchronize()  # the timed steps start and end on an idle device
     for _ in range(steps):
         e0.record(stream)
         prover.prove_device(sid, b.h_fft, b.poly, b.rows, stream=stream.handle)
@@ -99,6 +100,8 @@ def time_fused(b, prover, sid, steps, warmup):
         step_ms.append(e0.elapsed_ms(e1))
         for kind, ms in prover.launch_times():
             launches.setdefault(LAUNCH_KINDS[kind], []).append(ms)
+    stream.synchronize()
+    L.synchronize()
     prover.set_launch_timing(False)
     # the same sequence replayed from a hipGraph
     prover.set_graph(True)
@@ -342,6 +345,9 @@ def main():
     ap.add_argument("--uneven-rows", type=int, default=512,
                     help="rows per size in the uneven N=1024/2048/4096 batch (0: skip)")
     ap.add_argument("--workload", default="batch", choices=["batch", "serve"])
+    ap.add_argument("--calibrate-copy-mb", type=int, default=0,
+                    help="before timing, one device-to-device copy of this many MiB (the known-size "
+                         "kernel that scales FETCH_SIZE / WRITE_SIZE counter runs)")
     ap.add_argument("--rates", default="1000,2000,4000,6000,8000,10000,14000",
                     help="serve: offered loads in rows/s")
     ap.add_argument("--duration", type=float, default=2.0, help="serve: seconds per load level")
@@ -363,6 +369,16 @@ def main():
     if args.workload == "serve":
         bench_serve(args)
         return
+
+    if args.calibrate_copy_mb:
+        nb = args.calibrate_copy_mb << 20
+        src, dst = L.DeviceBuffer(nb), L.DeviceBuffer(nb)
+        src.fill(1)
+        L.call("fk20_memcpy_d2d", dst.ptr, src.ptr, nb)
+        L.synchronize()
+        src.free()
+        dst.free()
+        log("calibration copy: %d MiB device to device" % args.calibrate_copy_mb)
 
     log("generating %d rows, N=%d, setup on device" % (args.rows, args.N))
     b = Batch(args.rows, args.N, args.seed)

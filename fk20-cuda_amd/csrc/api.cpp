@@ -4,6 +4,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstring>
+#include <memory>
 #include <mutex>
 #include <vector>
 
@@ -42,7 +43,8 @@ struct DeviceState {
 };
 
 std::mutex g_mu;
-std::vector<DeviceState> g_dev;
+// one heap-allocated entry per device, so pointers handed out stay valid when the table grows
+std::vector<std::unique_ptr<DeviceState>> g_dev;
 
 }  // namespace
 
@@ -104,7 +106,8 @@ int device_state(DeviceState **out) {
     FK20_TRY(hipGetDevice(&dev));
     std::lock_guard<std::mutex> lk(g_mu);
     if ((int)g_dev.size() <= dev) g_dev.resize(dev + 1);
-    DeviceState &st = g_dev[dev];
+    if (!g_dev[dev]) g_dev[dev].reset(new DeviceState());
+    DeviceState &st = *g_dev[dev];
     if (!st.tables) {
         DevTables h;
         build_host_tables(h);
@@ -348,6 +351,10 @@ int fk20_memcpy_d2h(void *dst, const void *src, size_t bytes) {
 }
 
 int fk20_memset(void *dst, int value, size_t bytes) { return hip_status(hipMemset(dst, value, bytes)); }
+
+int fk20_memcpy_d2d(void *dst, const void *src, size_t bytes) {
+    return hip_status(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToDevice));
+}
 
 int fk20_synchronize(void) { return hip_status(hipDeviceSynchronize()); }
 

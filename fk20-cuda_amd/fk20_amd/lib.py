@@ -79,6 +79,7 @@ _SIGS = {
     "fk20_memcpy_h2d": ([_P, _P, _SZ], _I),
     "fk20_memcpy_d2h": ([_P, _P, _SZ], _I),
     "fk20_memset": ([_P, _I, _SZ], _I),
+    "fk20_memcpy_d2d": ([_P, _P, _SZ], _I),
     "fk20_synchronize": ([], _I),
     "fk20_stream_create": ([ctypes.POINTER(_P)], _I),
     "fk20_stream_destroy": ([_P], _I),

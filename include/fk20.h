@@ -219,6 +219,7 @@ int fk20_free(void *ptr);
 int fk20_memcpy_h2d(void *dst, const void *src, size_t bytes);
 int fk20_memcpy_d2h(void *dst, const void *src, size_t bytes);
 int fk20_memset(void *dst, int value, size_t bytes);
+int fk20_memcpy_d2d(void *dst, const void *src, size_t bytes);
 int fk20_synchronize(void);
 int fk20_stream_create(fk20_stream_t *stream);
 int fk20_stream_destroy(fk20_stream_t stream);
