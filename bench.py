@@ -46,6 +46,30 @@ def log(msg):
     print("[bench] " + msg, file=sys.stderr, flush=True)
 
 
+TRAFFIC_JSON = os.path.join(ROOT, "profiles", "fk20_batch_hbm_traffic.json")
+
+
+def measured_traffic(kernel, N, rows, window):
+    """HBM bytes per launch of `kernel` from the calibrated FETCH_SIZE/WRITE_SIZE passes
+    (fk20_amd.profsum traffic), if they were taken on this workload; else None."""
+    kind_to_symbol = {"k_dit_stage": "fk20::k_dit_multi<4>", "k_dif_stage": "fk20::k_dif_multi<4>",
+                      "k_middle": "fk20::k_middle_multi<4>", "k_msm_comb": "fk20::k_msm_multi",
+                      "k_toeplitz_fft_T": "fk20::k_toeplitz_fft_multi", "k_fft_output": "fk20::k_output_multi"}
+    try:
+        with open(TRAFFIC_JSON) as f:
+            t = json.load(f)
+    except OSError:
+        return None
+    if t.get("config") != {"N": N, "rows": rows, "window": window}:
+        return None
+    k = t["kernels"].get(kind_to_symbol.get(kernel, ""))
+    if not k:
+        return None
+    return {"bytes": k["fetch_bytes"] + k["write_bytes"], "fetch_bytes": k["fetch_bytes"],
+            "write_bytes": k["write_bytes"], "fetch_scale": t["fetch_scale"], "write_scale": t["write_scale"],
+            "source": os.path.relpath(TRAFFIC_JSON, ROOT)}
+
+
 def pct(v, q):
     v = sorted(v)
     return v[min(len(v) - 1, int(round(q / 100.0 * (len(v) - 1))))]
@@ -424,10 +448,20 @@ def main():
         "unit": "Tmul32/s",
         "frac": round(achieved / peak, 4),
         "traffic": None,
+        "traffic_detail": None,
         "work": "executed Fp muls of the kernel's algorithm x 300 mul32 (12x32-bit CIOS convention)",
         "fp_mul_per_s": round(exe_fp / (avg_ms / 1e3), 1),
         "achieved_canonical": round(canon / (avg_ms / 1e3) / 1e12, 4),
     }
+
+    tr = measured_traffic(dom, args.N, args.rows, args.window)
+    if tr:
+        roofline["traffic"] = tr["bytes"]
+        # algorithmic bytes: a stage launch reads and writes every (position, row) point once
+        n2 = args.N // 8
+        alg = {"k_dit_stage": 2, "k_dif_stage": 2, "k_middle": 2}.get(dom, 0) * n2 * args.rows * 144
+        tr["algorithmic_bytes"] = alg or None
+        roofline["traffic_detail"] = tr
 
     forms = {"F_rows_per_s": round(value, 3),
              "F_graph_rows_per_s": round(args.rows / (statistics.median(graph_ms) / 1e3), 3),

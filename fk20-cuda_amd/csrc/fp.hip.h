@@ -71,6 +71,14 @@ FK20_HD void fe_add(fe &z, const fe &a, const fe &b) {
     fe_norm(z);
 }
 
+// z = a + b without carry propagation: lower limbs < 2^30.  Such a value may be ONE operand
+// of fe_mul or fe_mul2 (the column bounds below allow one operand with limbs < 2^30), or
+// the input of fe_neg / fe_mul_small, which normalise; nothing else.
+FK20_HD void fe_add_nn(fe &z, const fe &a, const fe &b) {
+#pragma unroll
+    for (int i = 0; i < FP_N; i++) z.l[i] = a.l[i] + b.l[i];
+}
+
 template <int K>
 FK20_HD const uint32_t *fe_kp() {
     static_assert(K == 2 || K == 4 || K == 8 || K == 16 || K == 32 || K == 64 || K == 128, "K");
@@ -84,6 +92,15 @@ FK20_HD void fe_sub(fe &z, const fe &a, const fe &b) {
     const uint32_t *kp = fe_kp<K>();
 #pragma unroll
     for (int i = 0; i < FP_N; i++) z.l[i] = a.l[i] + kp[i] - b.l[i];
+    fe_norm_signed(z);
+}
+
+// z = a + K p - b - c; requires value(b) + value(c) <= K p.  Bound: bound(a) + K p.
+template <int K>
+FK20_HD void fe_sub2(fe &z, const fe &a, const fe &b, const fe &c) {
+    const uint32_t *kp = fe_kp<K>();
+#pragma unroll
+    for (int i = 0; i < FP_N; i++) z.l[i] = a.l[i] + kp[i] - b.l[i] - c.l[i];
     fe_norm_signed(z);
 }
 
@@ -113,7 +130,9 @@ FK20_HD void fe_mul_small(fe &z, const fe &a) {
 // ---------------------------------------------------------------- Montgomery product
 
 // z = a b / 2^406 mod p (lazy).  Product scanning with the reduction interleaved
-// (the "FIPS" ordering): column k accumulates a_i b_{k-i} and m_i p_{k-i}.
+// (the "FIPS" ordering): column k accumulates a_i b_{k-i} and m_i p_{k-i}: at most 14
+// products below 2^59 (one operand may have lower limbs < 2^30, see fe_add_nn) and 14
+// below 2^58, so the 64-bit accumulator never overflows.
 FK20_HD void fe_mul(fe &z, const fe &a, const fe &b) {
     uint32_t m[FP_N];
     uint64_t acc = 0;
@@ -131,6 +150,40 @@ FK20_HD void fe_mul(fe &z, const fe &a, const fe &b) {
     for (int k = FP_N; k < 2 * FP_N - 1; k++) {
 #pragma unroll
         for (int i = k - FP_N + 1; i < FP_N; i++) acc += (uint64_t)a.l[i] * b.l[k - i];
+#pragma unroll
+        for (int i = k - FP_N + 1; i < FP_N; i++) acc += (uint64_t)m[i] * FP_P[k - i];
+        z.l[k - FP_N] = (uint32_t)acc & FP_MASK;
+        acc >>= 29;
+    }
+    z.l[FP_N - 1] = (uint32_t)acc;
+}
+
+// z = (a b + c d) / 2^406 mod p (lazy) with one shared reduction.  Column k then holds at
+// most 28 input products and 14 reduction products: with the lower limbs of a, b, c, d
+// below 2^29 (every normalised fe_* output) except one operand below 2^30 (fe_add_nn),
+// 14 * 2^59 + 28 * 2^58 < 2^63.9, so the accumulator never overflows.  Top limbs are
+// small (values < 2^12 p).  Output < 1.03 p when a b + c d < 2^20 p^2.
+FK20_HD void fe_mul2(fe &z, const fe &a, const fe &b, const fe &c, const fe &d) {
+    uint32_t m[FP_N];
+    uint64_t acc = 0;
+#pragma unroll
+    for (int k = 0; k < FP_N; k++) {
+#pragma unroll
+        for (int i = 0; i <= k; i++) acc += (uint64_t)a.l[i] * b.l[k - i];
+#pragma unroll
+        for (int i = 0; i <= k; i++) acc += (uint64_t)c.l[i] * d.l[k - i];
+#pragma unroll
+        for (int i = 0; i < k; i++) acc += (uint64_t)m[i] * FP_P[k - i];
+        m[k] = ((uint32_t)acc * FP_PINV) & FP_MASK;
+        acc += (uint64_t)m[k] * FP_P[0];
+        acc >>= 29;
+    }
+#pragma unroll
+    for (int k = FP_N; k < 2 * FP_N - 1; k++) {
+#pragma unroll
+        for (int i = k - FP_N + 1; i < FP_N; i++) acc += (uint64_t)a.l[i] * b.l[k - i];
+#pragma unroll
+        for (int i = k - FP_N + 1; i < FP_N; i++) acc += (uint64_t)c.l[i] * d.l[k - i];
 #pragma unroll
         for (int i = k - FP_N + 1; i < FP_N; i++) acc += (uint64_t)m[i] * FP_P[k - i];
         z.l[k - FP_N] = (uint32_t)acc & FP_MASK;

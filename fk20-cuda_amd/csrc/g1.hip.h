@@ -7,7 +7,8 @@
 // (0 : 1 : 0); any (0 : y != 0 : 0) is accepted as infinity (g1.cuh:14 G1P_ANYINF).
 //
 // Value bounds (multiples of p) are annotated per step.  Inputs may be anything
-// below 10 p (every packed value is < 2^384 < 9.9 p); outputs are below 4 p.  The
+// below 10 p (every packed value is < 2^384 < 9.9 p; negated y < 16 p and additions
+// accept up to 20 p); outputs are below 4 p.  The
 // subtrahend bound decides each fe_sub<K> constant; every product has a*b < 2^20 p^2.
 #pragma once
 #include "fp.hip.h"
@@ -30,99 +31,80 @@ FK20_HD void g1_inf(g1p &p) {
     fe_zero(p.z);
 }
 
-// RCB algorithm 7: r = p + q.  Inputs < 10 p, output < 4 p.
+// RCB algorithm 7: r = p + q.  Inputs < 20 p, output < 2 p.  The three closing
+// sums of products (X3, Y3, Z3) each share one Montgomery reduction (fe_mul2).
 FK20_HD void g1_add(g1p &r, const g1p &p, const g1p &q) {
     fe t0, t1, t2, t3, t4, X3, Y3, Z3;
     fe_mul(t0, p.x, q.x);           // t0 < 2p
     fe_mul(t1, p.y, q.y);           // t1 < 2p
     fe_mul(t2, p.z, q.z);           // t2 < 2p
-    fe_add(t3, p.x, p.y);           // < 20p
-    fe_add(t4, q.x, q.y);           // < 20p
-    fe_mul(t3, t3, t4);             // < 2p
-    fe_add(t4, t0, t1);             // < 4p
-    fe_sub<4>(t3, t3, t4);          // < 6p
-    fe_add(t4, p.y, p.z);           // < 20p
-    fe_add(X3, q.y, q.z);           // < 20p
+    fe_add_nn(t3, p.x, p.y);        // < 40p, limbs < 2^30
+    fe_add(t4, q.x, q.y);           // < 40p
+    fe_mul(t3, t3, t4);             // 1600p^2 -> < 2p
+    fe_sub2<4>(t3, t3, t0, t1);     // X1 Y2 + X2 Y1 < 6p
+    fe_add_nn(t4, p.y, p.z);        // < 40p, limbs < 2^30
+    fe_add(X3, q.y, q.z);           // < 40p
     fe_mul(t4, t4, X3);             // < 2p
-    fe_add(X3, t1, t2);             // < 4p
-    fe_sub<4>(t4, t4, X3);          // < 6p
-    fe_add(X3, p.x, p.z);           // < 20p
-    fe_add(Y3, q.x, q.z);           // < 20p
+    fe_sub2<4>(t4, t4, t1, t2);     // Y1 Z2 + Y2 Z1 < 6p
+    fe_add_nn(X3, p.x, p.z);        // < 40p, limbs < 2^30
+    fe_add(Y3, q.x, q.z);           // < 40p
     fe_mul(X3, X3, Y3);             // < 2p
-    fe_add(Y3, t0, t2);             // < 4p
-    fe_sub<4>(Y3, X3, Y3);          // < 6p
-    fe_add(X3, t0, t0);             // < 4p
-    fe_add(t0, X3, t0);             // < 6p
+    fe_sub2<4>(Y3, X3, t0, t2);     // X1 Z2 + X2 Z1 < 6p
+    fe_mul_small<3>(t0, t0);        // < 6p
     fe_mul_small<12>(t2, t2);       // < 24p
-    fe_add(Z3, t1, t2);             // < 26p
+    fe_add_nn(Z3, t1, t2);          // < 26p, limbs < 2^30
     fe_sub<32>(t1, t1, t2);         // < 34p
     fe_mul_small<12>(Y3, Y3);       // < 72p
-    fe_mul(X3, t4, Y3);             // 6p*72p -> < 2p
-    fe_mul(t2, t3, t1);             // < 2p
-    fe_sub<2>(X3, t2, X3);          // < 4p
-    fe_mul(Y3, Y3, t0);             // < 2p
-    fe_mul(t1, t1, Z3);             // < 2p
-    fe_add(Y3, t1, Y3);             // < 4p
-    fe_mul(t0, t0, t3);             // < 2p
-    fe_mul(Z3, Z3, t4);             // < 2p
-    fe_add(Z3, Z3, t0);             // < 4p
+    fe_neg<8>(X3, t4);              // -t4 < 8p
+    fe_mul2(X3, t3, t1, X3, Y3);    // t3 t1 - t4 Y3: 6*34 + 8*72 = 780p^2 -> < 2p
+    fe_mul2(Y3, Y3, t0, t1, Z3);    // 72*6 + 34*26 = 1316p^2 -> < 2p (Z3 the one nn operand)
+    fe_mul2(Z3, Z3, t4, t0, t3);    // 26*6 + 6*6 = 192p^2 -> < 2p (Z3 the one nn operand)
     r.x = X3;
     r.y = Y3;
     r.z = Z3;
 }
 
-// RCB algorithm 8: r = p + (x2, y2), affine operand not infinity.  Inputs < 10 p.
+// RCB algorithm 8: r = p + (x2, y2), affine operand not infinity.  Inputs < 10 p,
+// output < 2 p.
 FK20_HD void g1_add_aff(g1p &r, const g1p &p, const fe &X2, const fe &Y2) {
     fe t0, t1, t2, t3, t4, X3, Y3, Z3;
     fe_mul(t0, p.x, X2);            // < 2p
     fe_mul(t1, p.y, Y2);            // < 2p
-    fe_add(t3, X2, Y2);             // < 20p
+    fe_add_nn(t3, X2, Y2);          // < 20p, limbs < 2^30
     fe_add(t4, p.x, p.y);           // < 20p
     fe_mul(t3, t3, t4);             // < 2p
-    fe_add(t4, t0, t1);             // < 4p
-    fe_sub<4>(t3, t3, t4);          // < 6p
+    fe_sub2<4>(t3, t3, t0, t1);     // < 6p
     fe_mul(t4, Y2, p.z);            // < 2p
-    fe_add(t4, t4, p.y);            // < 12p
+    fe_add_nn(t4, t4, p.y);         // < 12p, limbs < 2^30
     fe_mul(Y3, X2, p.z);            // < 2p
-    fe_add(Y3, Y3, p.x);            // < 12p
-    fe_add(X3, t0, t0);             // < 4p
-    fe_add(t0, X3, t0);             // < 6p
+    fe_add_nn(Y3, Y3, p.x);         // < 12p, limbs < 2^30
+    fe_mul_small<3>(t0, t0);        // < 6p
     fe_mul_small<12>(t2, p.z);      // < 120p
     fe_add(Z3, t1, t2);             // < 122p
     fe_sub<128>(t1, t1, t2);        // < 130p
-    fe_mul_small<12>(Y3, Y3);       // < 144p
-    fe_mul(X3, t4, Y3);             // 12p*144p -> < 2p
-    fe_mul(t2, t3, t1);             // 6p*130p  -> < 2p
-    fe_sub<2>(X3, t2, X3);          // < 4p
-    fe_mul(Y3, Y3, t0);             // 144p*6p  -> < 2p
-    fe_mul(t1, t1, Z3);             // 130p*122p -> < 2p
-    fe_add(Y3, t1, Y3);             // < 4p
-    fe_mul(t0, t0, t3);             // < 2p
-    fe_mul(Z3, Z3, t4);             // 122p*12p -> < 2p
-    fe_add(Z3, Z3, t0);             // < 4p
+    fe_mul_small<12>(Y3, Y3);       // < 144p (normalised)
+    fe_neg<16>(X3, t4);             // -t4 < 16p (normalised)
+    fe_mul2(X3, t3, t1, X3, Y3);    // 6*130 + 16*144 = 3084p^2 -> < 2p
+    fe_mul2(Y3, Y3, t0, t1, Z3);    // 144*6 + 130*122 = 16724p^2 -> < 2p
+    fe_mul2(Z3, Z3, t4, t0, t3);    // 122*12 + 6*6 = 1500p^2 -> < 2p (t4 the one nn operand)
     r.x = X3;
     r.y = Y3;
     r.z = Z3;
 }
 
-// RCB algorithm 9: r = 2p.  Input < 10 p, output < 4 p.
+// RCB algorithm 9: r = 2p.  Input < 20 p, output X < 4 p, Y, Z < 2 p.
 FK20_HD void g1_dbl(g1p &r, const g1p &p) {
     fe t0, t1, t2, X3, Y3, Z3;
-    fe_sqr(t0, p.y);                // < 2p
-    fe_add(Z3, t0, t0);             // < 4p
-    fe_add(Z3, Z3, Z3);             // < 8p
-    fe_add(Z3, Z3, Z3);             // < 16p
+    fe_sqr(t0, p.y);                // Y^2 < 2p
+    fe_mul_small<8>(Z3, t0);        // 8 Y^2 < 16p
     fe_mul(t1, p.y, p.z);           // < 2p
-    fe_sqr(t2, p.z);                // < 2p
-    fe_mul_small<12>(t2, t2);       // < 24p
-    fe_mul(X3, t2, Z3);             // < 2p
-    fe_add(Y3, t0, t2);             // < 26p
+    fe_sqr(X3, p.z);                // Z^2 < 2p
+    fe_mul_small<12>(t2, X3);       // 3b Z^2 < 24p
+    fe_mul_small<36>(X3, X3);       // 3 t2 < 72p
+    fe_add_nn(Y3, t0, t2);          // < 26p, limbs < 2^30
+    fe_sub<128>(t0, t0, X3);        // Y^2 - 3 t2 < 130p
+    fe_mul2(Y3, t2, Z3, t0, Y3);    // t2 Z3 + t0 Y3: 24*16 + 130*26 = 3764p^2 -> < 2p
     fe_mul(Z3, t1, Z3);             // < 2p
-    fe_add(t1, t2, t2);             // < 48p
-    fe_add(t2, t1, t2);             // < 72p
-    fe_sub<128>(t0, t0, t2);        // < 130p
-    fe_mul(Y3, t0, Y3);             // 130p*26p -> < 2p
-    fe_add(Y3, X3, Y3);             // < 4p
     fe_mul(t1, p.x, p.y);           // < 2p
     fe_mul(X3, t0, t1);             // < 2p
     fe_add(X3, X3, X3);             // < 4p

@@ -1,0 +1,117 @@
+"""Summaries of rocprofv3 output for the profiles/ folder.
+
+`stats`: the kernel-stats CSV of a `rocprofv3 --kernel-trace --stats -f csv` run as a
+Markdown table, with the command that made it.
+
+`traffic`: HBM traffic per kernel launch from two counter passes (FETCH_SIZE, WRITE_SIZE).
+Each pass is a `rocprofv3 --pmc <counter> -f csv` run of `bench.py --calibrate-copy-mb M`,
+which copies M MiB device to device before the timed work.  The counters' readings for that
+plain copy (M MiB read, M MiB written) give one scale factor per counter; every kernel's mean
+reading per dispatch is multiplied by it.  Writes a JSON summary (read by bench.py for
+`roofline.traffic`) and a Markdown table.
+
+    python3 -m fk20_amd.profsum traffic --fetch F.csv --write W.csv --copy-mb 256 \
+        --config N=4096,rows=512,window=12 --json out.json --md out.md --command "..."
+    python3 -m fk20_amd.profsum stats --csv run_kernel_stats.csv --title "..." \
+        --command "..." --md out.md [--note "..."]
+"""
+import argparse
+import collections
+import csv
+import json
+
+COPY_KERNEL = "__amd_rocclr_copyBuffer"
+
+
+def _kernel(name):
+    name = name.split("(")[0]
+    return name[5:] if name.startswith("void ") else name
+
+
+def read_pass(path, counter):
+    """{kernel: [KiB per dispatch]} for one counter pass."""
+    out = collections.defaultdict(list)
+    with open(path) as f:
+        for r in csv.DictReader(f):
+            if r["Counter_Name"] != counter:
+                continue
+            out[_kernel(r["Kernel_Name"])].append(float(r["Counter_Value"]))
+    return out
+
+
+def summarize(fetch, write, copy_mb):
+    copy_bytes = copy_mb << 20
+    # the calibration copy is the largest copy dispatch of the run
+    f_copy, w_copy = max(fetch[COPY_KERNEL]), max(write[COPY_KERNEL])
+    f_scale = copy_bytes / (f_copy * 1024)
+    w_scale = copy_bytes / (w_copy * 1024)
+    kernels = {}
+    for k in sorted(set(fetch) | set(write)):
+        if k.startswith("__amd_rocclr"):
+            continue
+        fv, wv = fetch.get(k, []), write.get(k, [])
+        kernels[k] = {
+            "dispatches": len(fv),
+            "fetch_bytes": round(sum(fv) / max(1, len(fv)) * 1024 * f_scale),
+            "write_bytes": round(sum(wv) / max(1, len(wv)) * 1024 * w_scale),
+        }
+    return {"fetch_scale": round(f_scale, 4), "write_scale": round(w_scale, 4),
+            "copy_fetch_kib": f_copy, "copy_write_kib": w_copy, "copy_mb": copy_mb,
+            "kernels": kernels}
+
+
+def stats_md(csv_path, title, command, note=""):
+    rows = list(csv.DictReader(open(csv_path)))
+    lines = ["# " + title, "", "Command (from the repository root on one MI355X):", "", "```", command.strip(), "```", ""]
+    if note:
+        lines += [note.strip(), ""]
+    lines += ["| kernel | calls | avg (ms) | min (ms) | max (ms) | total (ms) | % |", "|---|---:|---:|---:|---:|---:|---:|"]
+    for r in rows:
+        lines.append("| `%s` | %s | %.3f | %.3f | %.3f | %.1f | %.2f |" % (
+            _kernel(r["Name"]), r["Calls"], float(r["AverageNs"]) / 1e6, float(r["MinNs"]) / 1e6,
+            float(r["MaxNs"]) / 1e6, float(r["TotalDurationNs"]) / 1e6, float(r["Percentage"])))
+    return "\n".join(lines) + "\n"
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    sub = ap.add_subparsers(dest="cmd", required=True)
+    t = sub.add_parser("traffic")
+    t.add_argument("--fetch", required=True)
+    t.add_argument("--write", required=True)
+    t.add_argument("--copy-mb", type=int, required=True)
+    t.add_argument("--config", required=True, help="workload, e.g. N=4096,rows=512,window=12")
+    t.add_argument("--json", required=True)
+    t.add_argument("--md", required=True)
+    t.add_argument("--command", default="", help="the rocprofv3 commands, for the record")
+    k = sub.add_parser("stats")
+    k.add_argument("--csv", required=True)
+    k.add_argument("--title", required=True)
+    k.add_argument("--command", required=True)
+    k.add_argument("--note", default="")
+    k.add_argument("--md", required=True)
+    args = ap.parse_args()
+    if args.cmd == "stats":
+        with open(args.md, "w") as f:
+            f.write(stats_md(args.csv, args.title, args.command, args.note))
+        return
+    s = summarize(read_pass(args.fetch, "FETCH_SIZE"), read_pass(args.write, "WRITE_SIZE"), args.copy_mb)
+    s["config"] = {k: int(v) for k, v in (x.split("=") for x in args.config.split(","))}
+    s["command"] = args.command
+    with open(args.json, "w") as f:
+        json.dump(s, f, indent=1, sort_keys=True)
+    lines = ["# HBM traffic per launch (rocprofv3 FETCH_SIZE / WRITE_SIZE, calibrated)", "",
+             "Workload: %s." % ", ".join("%s=%d" % kv for kv in sorted(s["config"].items())), "",
+             "Calibration: a %d MiB device-to-device copy read FETCH_SIZE = %.1f KiB and WRITE_SIZE = "
+             "%.1f KiB, so FETCH_SIZE is scaled by %.4f and WRITE_SIZE by %.4f." %
+             (args.copy_mb, s["copy_fetch_kib"], s["copy_write_kib"], s["fetch_scale"], s["write_scale"]), "",
+             "| kernel | dispatches | fetch MB / launch | write MB / launch |", "|---|---:|---:|---:|"]
+    for name, v in s["kernels"].items():
+        lines.append("| `%s` | %d | %.2f | %.2f |" % (name, v["dispatches"], v["fetch_bytes"] / 1e6, v["write_bytes"] / 1e6))
+    lines += ["", "Commands:", "", "```", args.command.strip(), "```", ""]
+    with open(args.md, "w") as f:
+        f.write("\n".join(lines))
+
+
+if __name__ == "__main__":
+    main()

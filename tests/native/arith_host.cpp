@@ -76,6 +76,38 @@ void host_g1_add(uint64_t *r, const uint64_t *p, const uint64_t *q) {
     stp(r, a);
 }
 
+// Raw limbs in and out (14 x u32 each, Montgomery domain untouched): z = (ab + cd)/2^406
+// mod p, as fe_mul2 computes it, so limb extremes can be driven directly.
+void host_fp_mul2_raw(uint32_t *z, const uint32_t *a, const uint32_t *b, const uint32_t *c,
+                      const uint32_t *d) {
+    fe x, y, u, v, r;
+    memcpy(x.l, a, sizeof x.l);
+    memcpy(y.l, b, sizeof y.l);
+    memcpy(u.l, c, sizeof u.l);
+    memcpy(v.l, d, sizeof v.l);
+    fe_mul2(r, x, y, u, v);
+    memcpy(z, r.l, sizeof r.l);
+}
+
+void host_fp_mul_raw(uint32_t *z, const uint32_t *a, const uint32_t *b) {
+    fe x, y, r;
+    memcpy(x.l, a, sizeof x.l);
+    memcpy(y.l, b, sizeof y.l);
+    fe_mul(r, x, y);
+    memcpy(z, r.l, sizeof r.l);
+}
+
+// r = p + (x, y) with the affine operand given as a g1p_t whose z is ignored.
+void host_g1_add_aff(uint64_t *r, const uint64_t *p, const uint64_t *q) {
+    g1p a;
+    fe x, y;
+    ldp(a, p);
+    ld(x, q);
+    ld(y, q + 6);
+    g1_add_aff(a, a, x, y);
+    stp(r, a);
+}
+
 void host_g1_dbl(uint64_t *r, const uint64_t *p) {
     g1p a;
     ldp(a, p);

@@ -66,6 +66,41 @@ def test_fp_lazy_chains(L):
         assert fp_call(L, 3, a, None, reps=20) == pow(a, 2**20, P)
 
 
+def _limbs_int(v):
+    return sum(int(x) << (29 * k) for k, x in enumerate(v))
+
+
+def test_fp_mul_limb_extremes(L):
+    """fe_mul and fe_mul2 (two products, one reduction) are exact with limbs at their maxima:
+    lower limbs 2^29 - 1, top limbs at the 2^12 p value bound, and one operand carrying
+    unpropagated sums (fe_add_nn: lower limbs up to 2^30 - 1), where the column sums peak."""
+    rng = np.random.default_rng(13)
+    RINV = pow(2, -406, P)
+    top = (2**12 * P) >> (29 * 13)
+    full = [(1 << 29) - 1] * 13 + [top]
+    nn = [(1 << 30) - 1] * 13 + [2 * top]
+
+    def rand(bits):
+        return [int(x) for x in rng.integers(0, 1 << bits, 13)] + [int(rng.integers(0, top + 1))]
+    quads = [[nn, full, full, full], [full, full, full, full], [full, full, nn, full]]
+    quads += [[rand(30), rand(29), rand(29), rand(29)] for _ in range(60)]
+    quads += [[rand(29), rand(29), rand(29), rand(30)] for _ in range(60)]
+    z = np.zeros(14, dtype=np.uint32)
+    zp = z.ctypes.data_as(ctypes.c_void_p)
+    for q in quads:
+        ops = [np.array(c, dtype=np.uint32) for c in q]
+        a, b, c, d = [_limbs_int(o) for o in ops]
+        ptr = [o.ctypes.data_as(ctypes.c_void_p) for o in ops]
+        L.host_fp_mul2_raw(zp, *ptr)
+        zi = _limbs_int(z)
+        assert zi % P == (a * b + c * d) * RINV % P
+        assert zi < 2 * P and all(int(x) < (1 << 29) for x in z[:13])
+        L.host_fp_mul_raw(zp, ptr[0], ptr[1])
+        zi = _limbs_int(z)
+        assert zi % P == a * b * RINV % P
+        assert zi < 2 * P and all(int(x) < (1 << 29) for x in z[:13])
+
+
 def test_fr_mul(L):
     xs = S.random_fr(200, 5)
     ys = S.random_fr(200, 6)
@@ -86,6 +121,13 @@ def test_g1_formulas_against_oracle(L):
         r = np.zeros(18, dtype=np.uint64)
         L.host_g1_dbl(p64(r), p64(p))
         assert O.g1p_eq(r, O.g1p_op("dbl", p))
+    # mixed addition: affine right operand (never infinity), including q = +-p
+    for q in pts[:-1]:
+        qa = O.to_affine(q)
+        for p in pts:
+            r = np.zeros(18, dtype=np.uint64)
+            L.host_g1_add_aff(p64(r), p64(p), p64(qa))
+            assert O.g1p_eq(r, O.g1p_op("add", p, q))
     for k in S.random_fr(4, 9):
         r = np.zeros(18, dtype=np.uint64)
         L.host_g1_mul(p64(r), p64(G), p64(k))
