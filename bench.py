@@ -239,6 +239,87 @@ def time_uneven(prover, rows_each, window, steps, seed):
 
 # ---------------------------------------------------------------- C5: serving loop
 
+# ---------------------------------------------------------------- C3: subkernel microbench
+
+def random_points(count, seed):
+    """count points k*G (k uniform mod r) with a random Z (non-normalised), on the device."""
+    gens = L.DeviceBuffer.from_array(np.tile(G1_GEN, (count, 1)))
+    ks = L.DeviceBuffer.from_array(S.random_fr(count, seed))
+    pts = L.DeviceBuffer(count * 144)
+    L.call("fk20_g1p_op", 3, pts.ptr, gens.ptr, None, ks.ptr, count, None)
+    z = L.DeviceBuffer.from_array(np.repeat(S.random_fp_nonzero(count, seed + 1), 3, axis=0))
+    L.call("fk20_fp_op", 2, pts.ptr, pts.ptr, z.ptr, 3 * count, None)  # (X, Y, Z) * z
+    L.synchronize()
+    for b in (gens, ks, z):
+        b.free()
+    return pts
+
+
+def time_calls(fn, steps, stream):
+    e0, e1 = L.Event(), L.Event()
+    fn()
+    stream.synchronize()
+    ms = []
+    for _ in range(steps):
+        e0.record(stream)
+        fn()
+        e1.record(stream)
+        ms.append(e0.elapsed_ms(e1))
+    stream.synchronize()
+    e0.destroy()
+    e1.destroy()
+    return ms
+
+
+def bench_micro(args):
+    """C3: the drop-in subkernels at their FK20 shapes -- rows*16 Fr NTTs of size n2 on
+    uniform Fr, rows G1 FFT/IFFTs of n2 points k*G with random Z, and fk20_msm over n2
+    columns x 16 terms (uniform scalars, random projective points) at `rows` rows (throughput)
+    and at one row (latency)."""
+    rows, n2 = args.rows, args.N // 8
+    lg = int(np.log2(n2))
+    stream = L.Stream()
+    s = stream.handle
+    out = {}
+    log("micro: %d x 16 Fr NTT-%d" % (rows, n2))
+    fr = L.DeviceBuffer.from_array(S.random_fr(rows * 16 * n2, args.seed))
+    for name in ("fr_fft", "fr_ift"):
+        ms = statistics.median(time_calls(lambda: L.call(name, fr.ptr, fr.ptr, rows * 16, n2, s), args.steps, stream))
+        fr_muls = rows * 16 * (n2 // 2) * lg
+        out[name] = {"count": rows * 16, "n": n2, "ms_p50": round(ms, 4),
+                     "transforms_per_s": round(rows * 16 / (ms / 1e3), 1),
+                     "fr_mul_per_s": round(fr_muls / (ms / 1e3), 1)}
+    log("micro: %d G1 FFT/IFFT-%d on k*G with random Z" % (rows, n2))
+    pts = random_points(rows * n2, args.seed + 2)
+    res = L.DeviceBuffer(rows * n2 * 144)
+    for name in ("g1p_fft", "g1p_ift"):
+        ms = statistics.median(time_calls(lambda: L.call(name, res.ptr, pts.ptr, rows, n2, s), args.steps, stream))
+        canon = rows * (WL.stage_fp_muls(args.N)["fft" if name == "g1p_fft" else "ifft"]) * WL.FP_MUL32
+        out[name] = {"count": rows, "n": n2, "ms_p50": round(ms, 3),
+                     "transforms_per_s": round(rows / (ms / 1e3), 2),
+                     "butterflies_per_s": round(rows * (n2 // 2) * lg / (ms / 1e3), 1),
+                     "canonical_Tmul32_per_s": round(canon / (ms / 1e3) / 1e12, 3)}
+    pts.free()
+    log("micro: fk20_msm %d columns x 16 terms, rows %d and 1" % (n2, rows))
+    tc = L.DeviceBuffer.from_array(S.random_fr(rows * 16 * n2, args.seed + 3))
+    bases = random_points(16 * n2, args.seed + 4)
+    for r in (rows, 1):
+        ms = statistics.median(time_calls(lambda: L.call("fk20_msm", res.ptr, tc.ptr, bases.ptr, r, n2, s),
+                                          max(2, args.steps // 2) if r > 1 else args.steps, stream))
+        canon = r * WL.stage_fp_muls(args.N)["msm"] * WL.FP_MUL32
+        out["fk20_msm_rows%d" % r] = {"rows": r, "columns": n2, "terms": 16, "ms_p50": round(ms, 3),
+                                      "terms_per_s": round(r * 16 * n2 / (ms / 1e3), 1),
+                                      "canonical_Tmul32_per_s": round(canon / (ms / 1e3) / 1e12, 3)}
+    for b in (fr, res, tc, bases):
+        b.free()
+    stream.destroy()
+    print(json.dumps({"metric": "FK20 subkernel microbench (drop-in entry points)", "unit": "ms",
+                      "n_gpus": 1, "steps": args.steps, "higher_is_better": False,
+                      "data": "synthetic: uniform Fr, points k*G with random Z (seed %d)" % args.seed,
+                      "config": {"workload": "fk20_subkernels", "rows": rows, "N": args.N, "fft_size": n2},
+                      "kernels": out}))
+
+
 def bench_serve(args):
     """Poisson arrivals of one-row blobs, N = 1024/2048/4096 weighted 1:2:5, offered load swept."""
     import threading
@@ -368,7 +449,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--uneven-rows", type=int, default=512,
                     help="rows per size in the uneven N=1024/2048/4096 batch (0: skip)")
-    ap.add_argument("--workload", default="batch", choices=["batch", "serve"])
+    ap.add_argument("--workload", default="batch", choices=["batch", "serve", "micro"])
     ap.add_argument("--calibrate-copy-mb", type=int, default=0,
                     help="before timing, one device-to-device copy of this many MiB (the known-size "
                          "kernel that scales FETCH_SIZE / WRITE_SIZE counter runs)")
@@ -392,6 +473,9 @@ def main():
         args.window = tuned_window(args.N)
     if args.workload == "serve":
         bench_serve(args)
+        return
+    if args.workload == "micro":
+        bench_micro(args)
         return
 
     if args.calibrate_copy_mb:

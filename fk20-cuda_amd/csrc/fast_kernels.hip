@@ -282,14 +282,18 @@ __global__ void __launch_bounds__(64) k_msm_multi(Launch L) {
 template <int W>
 __device__ __forceinline__ void tw_mul(g1p &x, const uint32_t *ops) {
     constexpr int NT = 1 << (W - 2);
-    g1p T0 = x, T1, T2, T3;
+    // The odd multiples live in private (scratch) memory, not registers: each addition
+    // reads one entry, chosen by a wave-uniform index, so the loads are coalesced per
+    // dword and the ladder's register footprint stays within two waves per SIMD.
+    g1p T[NT];
+    T[0] = x;
     if (NT > 1) {
-        g1p x2;
+        g1p x2, c = x;
         g1_dbl(x2, x);
-        g1_add(T1, T0, x2);
-        if (NT > 2) {
-            g1_add(T2, T1, x2);
-            g1_add(T3, T2, x2);
+#pragma unroll
+        for (int k = 1; k < NT; k++) {
+            g1_add(c, c, x2);
+            T[k] = c;
         }
     }
     fe beta;
@@ -303,14 +307,8 @@ __device__ __forceinline__ void tw_mul(g1p &x, const uint32_t *ops) {
 #pragma unroll 1
         for (unsigned d = 0; d < nd; d++) g1_dbl(acc, acc);
         if (op & TWOP_END) break;
-        const unsigned e = (op >> TWOP_ENTRY_SHIFT) & 7;
-        g1p q;
-        switch (e) {
-        case 0: q = T0; break;
-        case 1: q = T1; break;
-        case 2: q = T2; break;
-        default: q = T3; break;
-        }
+        const unsigned e = (op >> TWOP_ENTRY_SHIFT) & (NT - 1);
+        g1p q = T[e];
         if (op & TWOP_PHI) fe_mul(q.x, q.x, beta);
         if (op & TWOP_NEG) fe_neg<16>(q.y, q.y);
         if (started) {
@@ -333,7 +331,7 @@ __device__ __forceinline__ void tw_mul(g1p &x, const uint32_t *ops) {
 
 // IFFT decimation in time (bit-reversed input), twiddles w^-idx before the butterfly.
 template <int W>
-__global__ void __launch_bounds__(64) k_dit_multi(Launch L, const uint32_t *ops, const uint32_t *index) {
+__global__ void __launch_bounds__(64, 2) k_dit_multi(Launch L, const uint32_t *ops, const uint32_t *index) {
     unsigned local;
     const Item &it = L.item[find_item(L, blockIdx.x, local)];
     const unsigned rows = it.rows, rg = (rows + 63) / 64, s = (unsigned)it.stage;
@@ -349,16 +347,16 @@ __global__ void __launch_bounds__(64) k_dit_multi(Launch L, const uint32_t *ops,
     if (idx) tw_mul<W>(x, ops + index[TW_INV * 512 + idx]);
     pt_load(y, D + ((size_t)l * rows + row) * 36);
     g1_add(a, y, x);
+    pt_store(D + ((size_t)l * rows + row) * 36, a);  // stored first: not live in the next add
     g1_neg(x, x);
     g1_add(x, y, x);
-    pt_store(D + ((size_t)l * rows + row) * 36, a);
     pt_store(D + ((size_t)r * rows + row) * 36, x);
 }
 
 // Last IFFT stage, upper half dropped (h = first half), then the first FFT DIF stage whose
 // upper input half is zero:  x = E_j + w^-idx O_j;  D[j] = x;  D[j + n2/2] = w^idx x.
 template <int W>
-__global__ void __launch_bounds__(64) k_middle_multi(Launch L, const uint32_t *ops, const uint32_t *index) {
+__global__ void __launch_bounds__(64, 2) k_middle_multi(Launch L, const uint32_t *ops, const uint32_t *index) {
     unsigned local;
     const Item &it = L.item[find_item(L, blockIdx.x, local)];
     const unsigned rows = it.rows, rg = (rows + 63) / 64, logn2 = it.logn2;
@@ -381,7 +379,7 @@ __global__ void __launch_bounds__(64) k_middle_multi(Launch L, const uint32_t *o
 // FFT decimation in frequency stage s >= 1 (span n2 >> (s+1)), twiddles w^idx after the
 // subtraction.
 template <int W>
-__global__ void __launch_bounds__(64) k_dif_multi(Launch L, const uint32_t *ops, const uint32_t *index) {
+__global__ void __launch_bounds__(64, 2) k_dif_multi(Launch L, const uint32_t *ops, const uint32_t *index) {
     unsigned local;
     const Item &it = L.item[find_item(L, blockIdx.x, local)];
     const unsigned rows = it.rows, rg = (rows + 63) / 64, logn2 = it.logn2, s = (unsigned)it.stage;

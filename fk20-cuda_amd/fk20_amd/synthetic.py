@@ -35,6 +35,31 @@ def random_fr(count, seed):
     return out
 
 
+P_MOD = 0x1A0111EA397FE69A4B1BA7B6434BACD764774B84F38512BF6730D2A0F6B0F6241EABFFFEB153FFFFB9FEFFFFFFFFAAAB
+_P_LIMBS = np.array([(P_MOD >> (64 * i)) & 0xFFFFFFFFFFFFFFFF for i in range(6)], dtype=np.uint64)
+
+
+def random_fp_nonzero(count, seed):
+    """`count` uniform non-zero Fp elements as a (count, 6) uint64 array (rejection sampling
+    of 381-bit draws)."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    out = np.empty((count, 6), dtype=np.uint64)
+    filled = 0
+    while filled < count:
+        need = count - filled
+        draw = rng.integers(0, 2**64, size=(need + need // 4 + 16, 6), dtype=np.uint64)
+        draw[:, 5] &= np.uint64((1 << 61) - 1)
+        lt = np.zeros(draw.shape[0], dtype=bool)
+        eq = np.ones(draw.shape[0], dtype=bool)
+        for i in range(5, -1, -1):
+            lt |= eq & (draw[:, i] < _P_LIMBS[i])
+            eq &= draw[:, i] == _P_LIMBS[i]
+        good = draw[lt & draw.any(axis=1)][:need]
+        out[filled : filled + good.shape[0]] = good
+        filled += good.shape[0]
+    return out
+
+
 def random_polys(rows, N, seed):
     """`rows` polynomials of N uniform Fr coefficients: (rows, N, 4)."""
     return random_fr(rows * N, seed).reshape(rows, N, 4)
