@@ -47,14 +47,22 @@ def log(msg):
 
 
 TRAFFIC_JSON = os.path.join(ROOT, "profiles", "fk20_batch_hbm_traffic.json")
+# launch kind (as timed by the library) -> kernel symbol prefix in rocprofv3 output
+KIND_SYMBOL = {"k_dit_stage": "fk20::k_dit_multi<", "k_dif_stage": "fk20::k_dif_multi<",
+               "k_middle": "fk20::k_middle_multi<", "k_msm_comb": "fk20::k_msm_multi",
+               "k_toeplitz_fft_T": "fk20::k_toeplitz_fft_multi", "k_fft_output": "fk20::k_output_multi",
+               "k_transform_chain": "fk20::k_chain<"}
+
+
+def _kernel_entry(kernels, kind):
+    prefix = KIND_SYMBOL.get(kind)
+    hits = [v for k, v in kernels.items() if prefix and k.startswith(prefix)]
+    return hits[0] if len(hits) == 1 else None
 
 
 def measured_traffic(kernel, N, rows, window):
     """HBM bytes per launch of `kernel` from the calibrated FETCH_SIZE/WRITE_SIZE passes
     (fk20_amd.profsum traffic), if they were taken on this workload; else None."""
-    kind_to_symbol = {"k_dit_stage": "fk20::k_dit_multi<4>", "k_dif_stage": "fk20::k_dif_multi<4>",
-                      "k_middle": "fk20::k_middle_multi<4>", "k_msm_comb": "fk20::k_msm_multi",
-                      "k_toeplitz_fft_T": "fk20::k_toeplitz_fft_multi", "k_fft_output": "fk20::k_output_multi"}
     try:
         with open(TRAFFIC_JSON) as f:
             t = json.load(f)
@@ -62,12 +70,38 @@ def measured_traffic(kernel, N, rows, window):
         return None
     if t.get("config") != {"N": N, "rows": rows, "window": window}:
         return None
-    k = t["kernels"].get(kind_to_symbol.get(kernel, ""))
+    k = _kernel_entry(t["kernels"], kernel)
     if not k:
         return None
     return {"bytes": k["fetch_bytes"] + k["write_bytes"], "fetch_bytes": k["fetch_bytes"],
             "write_bytes": k["write_bytes"], "fetch_scale": t["fetch_scale"], "write_scale": t["write_scale"],
             "source": os.path.relpath(TRAFFIC_JSON, ROOT)}
+
+
+VALU_JSON = os.path.join(ROOT, "profiles", "fk20_batch_valu_counters.json")
+
+
+def measured_valu(kernel, N, rows, window, avg_launch_s):
+    """Executed VALU instructions per launch of `kernel` from the SQ counter pass
+    (fk20_amd.profsum valu), if taken on this workload; the rate uses this run's own launch
+    time.  Else None."""
+    try:
+        with open(VALU_JSON) as f:
+            t = json.load(f)
+    except OSError:
+        return None
+    if t.get("config") != {"N": N, "rows": rows, "window": window}:
+        return None
+    k = _kernel_entry(t["kernels"], kernel)
+    if not k:
+        return None
+    lane_rate = k["valu_wave_instr"] * 64 / avg_launch_s
+    peak = WL.PEAK_MUL32_PER_S
+    return {"valu_wave_instr_per_launch": k["valu_wave_instr"], "int64_share": round(k["int64_share"], 4),
+            "lane_instr_per_s": round(lane_rate, 1), "issue_peak_per_s": peak,
+            "issue_frac": round(lane_rate / peak, 4),
+            "valu_busy": round(k["valu_busy"], 4) if k.get("valu_busy") is not None else None,
+            "source": os.path.relpath(VALU_JSON, ROOT)}
 
 
 def pct(v, q):
@@ -440,13 +474,15 @@ def main():
     ap.add_argument("--rows", type=int, default=512)
     ap.add_argument("--N", type=int, default=4096)
     ap.add_argument("--window", type=int, default=0,
-                    help="MSM window bits (8, 10, 12); 0: the autotuned choice (tuned.json) or 8")
+                    help="MSM window bits (8 to 13); 0: the autotuned choice (tuned.json) or 8")
     ap.add_argument("--seed", type=int, default=2024)
     ap.add_argument("--plain-steps", type=int, default=2, help="steps of the plain P2 form (0: skip)")
     ap.add_argument("--p1-rows", type=int, default=4, help="rows timed for the per-row form P1 (0: skip)")
     ap.add_argument("--cpu-rows", type=int, default=16, help="rows in the CPU baseline sample")
     ap.add_argument("--cpu-threads", type=int, default=16)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--chain", action="store_true",
+                    help="transform stages as one persistent dependency-driven launch (default: one launch per stage)")
     ap.add_argument("--uneven-rows", type=int, default=512,
                     help="rows per size in the uneven N=1024/2048/4096 batch (0: skip)")
     ap.add_argument("--workload", default="batch", choices=["batch", "serve", "micro"])
@@ -491,6 +527,7 @@ def main():
     log("generating %d rows, N=%d, setup on device" % (args.rows, args.N))
     b = Batch(args.rows, args.N, args.seed)
     prover = Prover()
+    prover.set_chain(args.chain)
     t0 = time.perf_counter()
     sid = prover.load_setup(b.xext, args.N, args.window)
     setup_s = time.perf_counter() - t0
@@ -533,17 +570,20 @@ def main():
         "frac": round(achieved / peak, 4),
         "traffic": None,
         "traffic_detail": None,
+        "valu_counters": None,
         "work": "executed Fp muls of the kernel's algorithm x 300 mul32 (12x32-bit CIOS convention)",
         "fp_mul_per_s": round(exe_fp / (avg_ms / 1e3), 1),
         "achieved_canonical": round(canon / (avg_ms / 1e3) / 1e12, 4),
     }
 
+    roofline["valu_counters"] = measured_valu(dom, args.N, args.rows, args.window, avg_ms / 1e3)
     tr = measured_traffic(dom, args.N, args.rows, args.window)
     if tr:
         roofline["traffic"] = tr["bytes"]
         # algorithmic bytes: a stage launch reads and writes every (position, row) point once
         n2 = args.N // 8
-        alg = {"k_dit_stage": 2, "k_dif_stage": 2, "k_middle": 2}.get(dom, 0) * n2 * args.rows * 144
+        alg = {"k_dit_stage": 2, "k_dif_stage": 2, "k_middle": 2,
+               "k_transform_chain": 2 * (2 * lg - 1)}.get(dom, 0) * n2 * args.rows * 144
         tr["algorithmic_bytes"] = alg or None
         roofline["traffic_detail"] = tr
 
@@ -591,7 +631,8 @@ def main():
         "dtype": "u32",
         "data": "synthetic: uniform Fr coefficients (PCG64 seed %d), setup from a seeded secret" % args.seed,
         "config": {"workload": "fk20_batch", "rows": args.rows, "N": args.N, "l": 16, "fft_size": args.N // 8,
-                   "form": "F batched (fk20_prove_rows)", "msm_window_bits": args.window},
+                   "form": "F batched (fk20_prove_rows)", "msm_window_bits": args.window,
+                   "transform": "persistent chain" if args.chain else "per-stage launches"},
         "latency_ms": {"step_p50": round(ms, 3), "step_p99": round(pct(step_ms, 99), 3)},
         "forms": forms,
         "uneven_batch": uneven,

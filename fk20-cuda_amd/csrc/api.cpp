@@ -309,6 +309,11 @@ int fk20_g1p_to_affine(fk20_g1a_t *out, const fk20_g1p_t *in, size_t count, fk20
     return hip_status(launch_to_affine(W(out), W(in), count, S(stream)));
 }
 
+int fk20_g1p_compress(uint8_t *out, const fk20_g1p_t *in, size_t count, fk20_stream_t stream) {
+    if (count && (!out || !in)) return FK20_EINVAL;
+    return hip_status(launch_compress(W(out), W(in), count, S(stream)));
+}
+
 int fk20_fp_op(int op, uint64_t *out, const uint64_t *a, const uint64_t *b, size_t count, fk20_stream_t stream) {
     if (op < 0 || op > 6 || (count && (!out || !a || (op <= 2 && !b)))) return FK20_EINVAL;
     return hip_status(launch_fp_op(op, W(out), W(a), W(b), count, S(stream)));

@@ -57,6 +57,7 @@ hipError_t launch_msm_plain(uint32_t *hext_fft, const uint32_t *tc_fft, const ui
                             unsigned rows, unsigned n2, hipStream_t s);
 hipError_t launch_setup_gather(uint32_t *xext, const uint32_t *srs, unsigned N, hipStream_t s);
 hipError_t launch_to_affine(uint32_t *out, const uint32_t *in, size_t count, hipStream_t s);
+hipError_t launch_compress(uint32_t *out, const uint32_t *in, size_t count, hipStream_t s);
 hipError_t launch_fp_op(int op, uint32_t *out, const uint32_t *a, const uint32_t *b, size_t count,
                         hipStream_t s);
 hipError_t launch_fr_op(int op, uint32_t *out, const uint32_t *a, const uint32_t *b, size_t count,

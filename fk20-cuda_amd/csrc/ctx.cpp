@@ -145,7 +145,7 @@ struct Setup {
 };
 
 bool valid_N(unsigned N) { return N == 1024 || N == 2048 || N == 4096; }
-bool valid_win(unsigned w) { return w == 8 || w == 10 || w == 12; }
+bool valid_win(unsigned w) { return w >= 8 && w <= 13; }
 unsigned nwin_of(unsigned w) { return (256 + w - 1) / w; }
 
 int status_of(hipError_t e) { return status_from_hip(e); }
@@ -160,6 +160,8 @@ struct fk20_ctx {
     void *work = nullptr;
     size_t work_bytes = 0;
     bool use_graph = false;
+    bool use_chain = false;  // transform stages as one persistent launch (k_chain)
+    int out_format = FK20_OUT_G1P;
     std::vector<fk20_batch_item_t> gkey;  // items of the captured graph
     hipStream_t gstream = nullptr;
     hipGraphExec_t gexec = nullptr;
@@ -188,10 +190,12 @@ size_t fk20_setup_table_bytes(unsigned N, unsigned window) {
     return (size_t)16 * (N / 8) * nwin_of(window) * ((size_t)1 << (window - 1)) * 96;
 }
 
+// Per item: tcT (16 n2 Fr per row), D (n2 points per row), the chain flags (n2 words per
+// 64-row group) and a 256-byte control block (the chain's work counter; item 0's is used).
 size_t fk20_rows_workspace_bytes(unsigned rows, unsigned N) {
     if (!valid_N(N)) return 0;
     const size_t n2 = N / 8;
-    return (size_t)rows * n2 * (16 * 32 + 144);
+    return (size_t)rows * n2 * (16 * 32 + 144) + (size_t)((rows + 63) / 64) * n2 * 4 + 256;
 }
 
 int fk20_ctx_create(fk20_ctx_t **out) {
@@ -224,6 +228,42 @@ int fk20_ctx_destroy(fk20_ctx_t *c) {
     return FK20_OK;
 }
 
+size_t fk20_output_point_bytes(int format) {
+    return format == FK20_OUT_G1P || format == FK20_OUT_G1P_DA ? 144
+         : format == FK20_OUT_COMPRESSED || format == FK20_OUT_COMPRESSED_DA ? 48 : 0;
+}
+
+int fk20_ctx_set_output(fk20_ctx_t *c, int format) {
+    if (!c || !fk20_output_point_bytes(format)) return FK20_EINVAL;
+    std::lock_guard<std::mutex> lk(c->mu);
+    if (c->out_format != format && c->gexec) {
+        (void)hipGraphExecDestroy(c->gexec);  // the format is captured in the launch arguments
+        c->gexec = nullptr;
+        c->gkey.clear();
+    }
+    c->out_format = format;
+    return FK20_OK;
+}
+
+int fk20_ctx_get_output(fk20_ctx_t *c, int *format) {
+    if (!c || !format) return FK20_EINVAL;
+    std::lock_guard<std::mutex> lk(c->mu);
+    *format = c->out_format;
+    return FK20_OK;
+}
+
+int fk20_ctx_set_chain(fk20_ctx_t *c, int enable) {
+    if (!c) return FK20_EINVAL;
+    std::lock_guard<std::mutex> lk(c->mu);
+    if (c->use_chain != (enable != 0) && c->gexec) {
+        (void)hipGraphExecDestroy(c->gexec);  // captured with the other launch sequence
+        c->gexec = nullptr;
+        c->gkey.clear();
+    }
+    c->use_chain = enable != 0;
+    return FK20_OK;
+}
+
 int fk20_ctx_set_graph(fk20_ctx_t *c, int enable) {
     if (!c) return FK20_EINVAL;
     std::lock_guard<std::mutex> lk(c->mu);
@@ -244,9 +284,10 @@ int fk20_setup_load(fk20_ctx_t *c, const fk20_g1p_t *xext_fft, unsigned N, unsig
     const unsigned nbases = 16 * st.n2, H = 1u << (window - 1);
     const unsigned logn2 = st.n2 == 128 ? 7 : st.n2 == 256 ? 8 : 9;
     uint32_t *xs = nullptr, *zt = nullptr;
-    // bases per build launch, bounding the batch-inversion scratch to ~512 MiB
+    // bases per build launch, bounding the batch-inversion scratch to ~2 GiB (one thread per
+    // (base, window) builds its 2^(w-1) entries, so larger chunks mean more parallel threads)
     const size_t per_base = (size_t)st.nwin * H * 96;
-    unsigned chunk = (unsigned)std::max<size_t>(1, ((size_t)512 << 20) / per_base);
+    unsigned chunk = (unsigned)std::max<size_t>(1, ((size_t)2 << 30) / per_base);
     if (chunk > nbases) chunk = nbases;
     hipError_t e = hipMalloc(&st.table, fk20_setup_table_bytes(N, window));
     if (e == hipSuccess) e = hipMalloc(&st.inf, nbases * 4);
@@ -333,6 +374,7 @@ static hipError_t enqueue_items(fk20_ctx *c, const fk20_batch_item_t *items, uns
     memset(&L, 0, sizeof L);
     L.nitems = n;
     char *w = reinterpret_cast<char *>(workspace ? workspace : c->work);
+    uint32_t *ctl = nullptr;
     for (unsigned k = 0; k < n; k++) {
         const Setup &st = c->setups[items[k].setup_id];
         Item &it = L.item[k];
@@ -343,15 +385,20 @@ static hipError_t enqueue_items(fk20_ctx *c, const fk20_batch_item_t *items, uns
         w += rows * 16 * st.n2 * 32;
         it.D = reinterpret_cast<uint32_t *>(w);
         w += rows * st.n2 * 144;
+        it.flags = reinterpret_cast<uint32_t *>(w);
+        w += (rows + 63) / 64 * st.n2 * 4;
+        if (!k) ctl = reinterpret_cast<uint32_t *>(w);
+        w += 256;
         it.table = st.table;
         it.inf = st.inf;
         it.rows = items[k].rows;
         it.N = st.N;
         it.logn2 = st.n2 == 128 ? 7 : st.n2 == 256 ? 8 : 9;
         it.win = st.win;
+        it.out_format = (uint32_t)c->out_format;
         it.stage = -1;
     }
-    return launch_batch(L, c->T, c->prog.ops, c->prog.index, s, hook);
+    return launch_batch(L, c->T, c->prog.ops, c->prog.index, s, hook, c->use_chain ? ctl : nullptr);
 }
 
 int fk20_prove_batch(fk20_ctx_t *c, const fk20_batch_item_t *items, unsigned nitems, fk20_stream_t stream) {
@@ -470,10 +517,11 @@ int fk20_ctx_launch_times(fk20_ctx_t *c, float *ms, int *kinds, int max, int *co
     return FK20_OK;
 }
 
-int fk20_twiddle_selfcheck(unsigned *total_ops) {
+int fk20_twiddle_selfcheck(unsigned *total_ops, unsigned *wnaf_width) {
     std::vector<uint32_t> ops, index;
     const bool ok = build_programs_host(ops, index);
     if (total_ops) *total_ops = (unsigned)ops.size();
+    if (wnaf_width) *wnaf_width = (unsigned)TW_WNAF;
     return ok ? FK20_OK : FK20_EINVAL;
 }
 

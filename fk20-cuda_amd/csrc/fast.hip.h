@@ -18,9 +18,10 @@ static constexpr unsigned TWOP_ENTRY_SHIFT = 9;
 static constexpr uint32_t TWOP_NEG = 1u << 12;
 static constexpr uint32_t TWOP_END = 1u << 31;
 
-// wNAF width used by the G1 transform kernels (odd multiples P, 3P, 5P, 7P)
+// wNAF width used by the G1 transform kernels (odd multiples P, 3P, .., 15P, kept in private
+// memory); the twiddle programs are built for the same width
 #ifndef FK20_TW_WNAF
-#define FK20_TW_WNAF 4
+#define FK20_TW_WNAF 5
 #endif
 static constexpr int TW_WNAF = FK20_TW_WNAF;
 // direction rows of the program index: index[dir * 512 + idx] = offset of w^(+-idx)'s program
@@ -38,9 +39,11 @@ struct Item {
     uint32_t *h_fft;        // [rows][n2] g1p_t
     uint32_t *tcT;          // workspace [n2][16][rows] fr_t
     uint32_t *D;            // workspace [n2][rows] packed Montgomery points
+    uint32_t *flags;        // workspace [ceil(rows/64)][n2]: last local stage written (chain)
     const uint32_t *table;  // setup window tables
     const uint32_t *inf;    // setup infinity flags
     uint32_t rows, N, logn2, win;
+    uint32_t out_format;    // FK20_OUT_* (bit 0: data-availability order, bit 1: compressed)
     int32_t stage;          // set per launch by launch_batch
 };
 struct Launch {
@@ -49,8 +52,21 @@ struct Launch {
     uint32_t nitems;
 };
 
+// The transform stages of a batch as one persistent launch (k_chain): work items are
+// numbered stage by stage; t_end[t] = items in global stages 0..t, blk_end[t][k] = items of
+// items 0..k within stage t, ustage[t][k] = item k's local stage at global stage t.
+static constexpr unsigned MAX_STAGES = 17;  // 2 * 9 - 1 for n2 = 512
+struct Chain {
+    Launch L;
+    uint32_t *ctl;  // [0] work counter, [1] error word (a bounded wait expired)
+    uint32_t total, nstages;
+    uint32_t t_end[MAX_STAGES];
+    uint32_t blk_end[MAX_STAGES][MAX_ITEMS];
+    int8_t ustage[MAX_STAGES][MAX_ITEMS];
+};
+
 // Kernel kinds of the batched path, reported by the per-launch timing of the context.
-enum LaunchKind { LK_TOEPLITZ_FFT = 0, LK_MSM = 1, LK_DIT = 2, LK_MIDDLE = 3, LK_DIF = 4, LK_OUTPUT = 5 };
+enum LaunchKind { LK_TOEPLITZ_FFT = 0, LK_MSM = 1, LK_DIT = 2, LK_MIDDLE = 3, LK_DIF = 4, LK_OUTPUT = 5, LK_CHAIN = 6 };
 // Called after each launch when the context times launches (fn == nullptr otherwise).
 struct LaunchHook {
     void (*fn)(void *arg, int kind);
@@ -62,7 +78,9 @@ struct LaunchHook {
 
 // The whole batched path for the items of `L`: Toeplitz + NTT, MSM, 2*Lmax - 1 transform
 // stages, output -- one launch per stage covering every item.
+// chain != nullptr: the transform stages run as one persistent k_chain launch whose work
+// counter lives at `chain` (two words, cleared by the launch sequence itself).
 hipError_t launch_batch(const Launch &L, const DevTables *T, const uint32_t *ops, const uint32_t *index,
-                        hipStream_t s, LaunchHook hook);
+                        hipStream_t s, LaunchHook hook, uint32_t *chain = nullptr);
 
 }  // namespace fk20

@@ -19,6 +19,8 @@
 //  6. k_fft_output     D[pos][row] -> h_fft[row][bitrev(pos)] in the reference ABI.
 // Same mathematics as fk20_msm.cu:23-66, g1p_fft.cu:25-330 and fk20_hext_fft2h_fft.cu:27-70;
 // the layout and algorithms are this library's own.
+#include <cstring>
+
 #include "common.hip.h"
 #include "fast.hip.h"
 
@@ -327,83 +329,191 @@ __device__ __forceinline__ void tw_mul(g1p &x, const uint32_t *ops) {
 //   IFFT DIT stage s (0 <= s <= L-2)  at launch Lmax-1-(L-1)+s   (right-aligned)
 //   middle                            at launch Lmax-1            (shared)
 //   FFT DIF stage s (1 <= s <= L-1)   at launch Lmax-1+s          (left-aligned)
-// so every item of a launch runs the same kind; it.stage holds s (or -1: idle).
+// so every item of a launch runs the same kind; it.stage holds s (or -1: idle).  Within an
+// item, local stage u = 0 .. 2L-2 numbers its own stages: DIT s = u (u < L-1), the middle
+// (u = L-1), DIF s = u - (L-1).  Butterfly b of a stage reads and writes two positions
+// (in place), given by stage_positions().
 
-// IFFT decimation in time (bit-reversed input), twiddles w^-idx before the butterfly.
+struct Butterfly {
+    unsigned p0, p1;  // positions read and written
+    unsigned idx;     // twiddle exponent (w^+-idx), 0: none
+};
+
+__device__ __forceinline__ Butterfly stage_positions(unsigned u, unsigned logn2, unsigned b) {
+    Butterfly f;
+    if (u + 1 < logn2) {  // IFFT DIT stage s = u: span 2^s
+        const unsigned s = u, j = b & ((1u << s) - 1);
+        f.p0 = ((b >> s) << (s + 1)) + j;
+        f.p1 = f.p0 + (1u << s);
+        f.idx = j << (8 - s);
+    } else if (u + 1 == logn2) {  // middle: positions j, j + n2/2
+        f.p0 = b;
+        f.p1 = b + (1u << (logn2 - 1));
+        f.idx = b << (9 - logn2);
+    } else {  // FFT DIF stage s = u - (L-1): span 2^(L-s-1)
+        const unsigned s = u + 1 - logn2, lh = logn2 - s - 1, j = b & ((1u << lh) - 1);
+        f.p0 = ((b >> lh) << (lh + 1)) + j;
+        f.p1 = f.p0 + (1u << lh);
+        f.idx = j << (8 - lh);
+    }
+    return f;
+}
+
+// IFFT decimation in time (bit-reversed input): x = D[p1] w^-idx; D[p0], D[p1] = y + x, y - x.
 template <int W>
-__global__ void __launch_bounds__(64, 2) k_dit_multi(Launch L, const uint32_t *ops, const uint32_t *index) {
-    unsigned local;
-    const Item &it = L.item[find_item(L, blockIdx.x, local)];
-    const unsigned rows = it.rows, rg = (rows + 63) / 64, s = (unsigned)it.stage;
-    const unsigned b = local / rg;
-    const unsigned row = (local % rg) * 64 + threadIdx.x;
-    if (row >= rows) return;
-    uint32_t *D = it.D;
-    const unsigned j = b & ((1u << s) - 1);
-    const unsigned l = ((b >> s) << (s + 1)) + j, r = l + (1u << s);
-    const unsigned idx = j << (8 - s);
+__device__ __forceinline__ void dit_butterfly(uint32_t *D, unsigned rows, unsigned row, const Butterfly &f,
+                                              const uint32_t *ops, const uint32_t *index) {
     g1p x, y, a;
-    pt_load(x, D + ((size_t)r * rows + row) * 36);
-    if (idx) tw_mul<W>(x, ops + index[TW_INV * 512 + idx]);
-    pt_load(y, D + ((size_t)l * rows + row) * 36);
+    pt_load(x, D + ((size_t)f.p1 * rows + row) * 36);
+    if (f.idx) tw_mul<W>(x, ops + index[TW_INV * 512 + f.idx]);
+    pt_load(y, D + ((size_t)f.p0 * rows + row) * 36);
     g1_add(a, y, x);
-    pt_store(D + ((size_t)l * rows + row) * 36, a);  // stored first: not live in the next add
+    pt_store(D + ((size_t)f.p0 * rows + row) * 36, a);  // stored first: not live in the next add
     g1_neg(x, x);
     g1_add(x, y, x);
-    pt_store(D + ((size_t)r * rows + row) * 36, x);
+    pt_store(D + ((size_t)f.p1 * rows + row) * 36, x);
 }
 
 // Last IFFT stage, upper half dropped (h = first half), then the first FFT DIF stage whose
 // upper input half is zero:  x = E_j + w^-idx O_j;  D[j] = x;  D[j + n2/2] = w^idx x.
 template <int W>
-__global__ void __launch_bounds__(64, 2) k_middle_multi(Launch L, const uint32_t *ops, const uint32_t *index) {
-    unsigned local;
-    const Item &it = L.item[find_item(L, blockIdx.x, local)];
-    const unsigned rows = it.rows, rg = (rows + 63) / 64, logn2 = it.logn2;
-    const unsigned j = local / rg;
-    const unsigned row = (local % rg) * 64 + threadIdx.x;
-    if (row >= rows) return;
-    uint32_t *D = it.D;
-    const unsigned hn = 1u << (logn2 - 1);
-    const unsigned idx = j << (9 - logn2);
+__device__ __forceinline__ void middle_butterfly(uint32_t *D, unsigned rows, unsigned row, const Butterfly &f,
+                                                 const uint32_t *ops, const uint32_t *index) {
     g1p o, e;
-    pt_load(o, D + ((size_t)(j + hn) * rows + row) * 36);
-    if (idx) tw_mul<W>(o, ops + index[TW_INV * 512 + idx]);
-    pt_load(e, D + ((size_t)j * rows + row) * 36);
+    pt_load(o, D + ((size_t)f.p1 * rows + row) * 36);
+    if (f.idx) tw_mul<W>(o, ops + index[TW_INV * 512 + f.idx]);
+    pt_load(e, D + ((size_t)f.p0 * rows + row) * 36);
     g1_add(e, e, o);
-    pt_store(D + ((size_t)j * rows + row) * 36, e);
-    if (idx) tw_mul<W>(e, ops + index[TW_FWD * 512 + idx]);
-    pt_store(D + ((size_t)(j + hn) * rows + row) * 36, e);
+    pt_store(D + ((size_t)f.p0 * rows + row) * 36, e);
+    if (f.idx) tw_mul<W>(e, ops + index[TW_FWD * 512 + f.idx]);
+    pt_store(D + ((size_t)f.p1 * rows + row) * 36, e);
 }
 
-// FFT decimation in frequency stage s >= 1 (span n2 >> (s+1)), twiddles w^idx after the
-// subtraction.
+// FFT decimation in frequency: D[p0], D[p1] = u + v, (u - v) w^idx.
 template <int W>
-__global__ void __launch_bounds__(64, 2) k_dif_multi(Launch L, const uint32_t *ops, const uint32_t *index) {
-    unsigned local;
-    const Item &it = L.item[find_item(L, blockIdx.x, local)];
-    const unsigned rows = it.rows, rg = (rows + 63) / 64, logn2 = it.logn2, s = (unsigned)it.stage;
-    const unsigned b = local / rg;
-    const unsigned row = (local % rg) * 64 + threadIdx.x;
-    if (row >= rows) return;
-    uint32_t *D = it.D;
-    const unsigned lh = logn2 - s - 1;  // log2 of the span
-    const unsigned h = 1u << lh;
-    const unsigned j = b & (h - 1);
-    const unsigned l = ((b >> lh) << (lh + 1)) + j, r = l + h;
-    const unsigned idx = j << (8 - lh);
+__device__ __forceinline__ void dif_butterfly(uint32_t *D, unsigned rows, unsigned row, const Butterfly &f,
+                                              const uint32_t *ops, const uint32_t *index) {
     g1p u, v, a;
-    pt_load(u, D + ((size_t)l * rows + row) * 36);
-    pt_load(v, D + ((size_t)r * rows + row) * 36);
+    pt_load(u, D + ((size_t)f.p0 * rows + row) * 36);
+    pt_load(v, D + ((size_t)f.p1 * rows + row) * 36);
     g1_add(a, u, v);
-    pt_store(D + ((size_t)l * rows + row) * 36, a);  // stored first: not live across tw_mul
+    pt_store(D + ((size_t)f.p0 * rows + row) * 36, a);  // stored first: not live across tw_mul
     g1_neg(v, v);
     g1_add(v, u, v);
-    if (idx) tw_mul<W>(v, ops + index[TW_FWD * 512 + idx]);
-    pt_store(D + ((size_t)r * rows + row) * 36, v);
+    if (f.idx) tw_mul<W>(v, ops + index[TW_FWD * 512 + f.idx]);
+    pt_store(D + ((size_t)f.p1 * rows + row) * 36, v);
 }
 
-// D[pos][row] (bit-reversed) -> h_fft[row][bitrev(pos)], canonical standard projective.
+// Runs local stage u's butterfly `local` of item `it` for this lane's row.  Lanes past the
+// item's last row repeat the last row's work (same loads, same stores of the same values),
+// so the wave never diverges: the persistent chain below relies on every lane staying
+// converged around its wave-uniform steps.
+template <int W>
+__device__ __forceinline__ void run_butterfly(const Item &it, unsigned u, unsigned local, const uint32_t *ops,
+                                              const uint32_t *index) {
+    const unsigned rows = it.rows, rg = (rows + 63) / 64;
+    const unsigned row = min((local % rg) * 64 + threadIdx.x, rows - 1);
+    const Butterfly f = stage_positions(u, it.logn2, local / rg);
+    if (u + 1 < it.logn2) dit_butterfly<W>(it.D, rows, row, f, ops, index);
+    else if (u + 1 == it.logn2) middle_butterfly<W>(it.D, rows, row, f, ops, index);
+    else dif_butterfly<W>(it.D, rows, row, f, ops, index);
+}
+
+// One launch per stage: it.stage = s of this launch's kind (DIT/DIF), 0 for the middle.
+// Each kernel inlines only its own butterfly (the chain kernel below needs all three).
+#define FK20_STAGE_KERNEL(NAME, BODY, USTAGE)                                                       \
+    template <int W>                                                                             \
+    __global__ void __launch_bounds__(64, 2) NAME(Launch L, const uint32_t *ops, const uint32_t *index) { \
+        unsigned local;                                                                          \
+        const Item &it = L.item[find_item(L, blockIdx.x, local)];                                \
+        const unsigned rg = (it.rows + 63) / 64, row = (local % rg) * 64 + threadIdx.x;          \
+        if (row >= it.rows) return;                                                              \
+        const Butterfly f = stage_positions((USTAGE), it.logn2, local / rg);                     \
+        BODY<W>(it.D, it.rows, row, f, ops, index);                                              \
+    }
+FK20_STAGE_KERNEL(k_dit_multi, dit_butterfly, (unsigned)it.stage)
+FK20_STAGE_KERNEL(k_middle_multi, middle_butterfly, it.logn2 - 1)
+FK20_STAGE_KERNEL(k_dif_multi, dif_butterfly, it.logn2 - 1 + (unsigned)it.stage)
+#undef FK20_STAGE_KERNEL
+
+// ---------------------------------------------------------------- persistent transform chain
+//
+// All 2*Lmax - 1 stages in one launch of resident waves.  Work items (one butterfly of one
+// 64-row group) are numbered stage by stage; a wave takes the next number from a counter,
+// waits until the two positions it reads carry the flag of the item's previous local stage,
+// runs the butterfly and flags both positions with u + 1.  A wave that finishes a cheap
+// (untwiddled) butterfly moves on to later stages at once instead of idling at a launch
+// boundary.  Items are taken in stage order, so every item an item depends on was taken
+// earlier by a running wave: the waits always end.  Each wait is bounded all the same, by
+// the constant-rate wall clock (CHAIN_WAIT_TICKS, about 1.3 s at the 100 MHz reference;
+// a legitimate wait is at most one pass, tens of ms); on expiry the wave records an error
+// in ctl[1] and proceeds.
+
+static constexpr uint64_t CHAIN_WAIT_TICKS = 1ull << 27;
+
+__device__ __forceinline__ unsigned flag_load(const uint32_t *f) {
+    return __hip_atomic_load(f, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+template <int W>
+__global__ void __launch_bounds__(64, 2) k_chain(Chain C, const uint32_t *ops, const uint32_t *index) {
+    // No lane-divergent branch anywhere in this loop: the work counter is bumped by lane 0's
+    // addend (every lane executes the atomic, the others add 0) and every lane stores the
+    // flags, so the wave-uniform values below always come from a fully converged wave.  A
+    // wave can take at most `total` items, which bounds the loop whatever happens.
+#pragma unroll 1
+    for (unsigned iter = 0; iter < C.total; iter++) {
+        const unsigned got =
+            __hip_atomic_fetch_add(C.ctl, threadIdx.x == 0 ? 1u : 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        const unsigned w = __builtin_amdgcn_readfirstlane(got);  // lane 0's: the value before its add
+        if (w >= C.total) break;
+        unsigned t = 0;
+#pragma unroll 1
+        while (w >= C.t_end[t]) t++;
+        unsigned local = w - (t ? C.t_end[t - 1] : 0);
+        unsigned k = 0;
+#pragma unroll 1
+        while (local >= C.blk_end[t][k]) k++;
+        if (k) local -= C.blk_end[t][k - 1];
+        const Item &it = C.L.item[k];
+        const unsigned u = (unsigned)C.ustage[t][k];
+        const unsigned rg = (it.rows + 63) / 64, g = local % rg;
+        const Butterfly f = stage_positions(u, it.logn2, local / rg);
+        uint32_t *fl = it.flags + (size_t)g * (1u << it.logn2);
+        if (u) {
+            const uint64_t t0 = wall_clock64();
+            bool expired = false;
+#pragma unroll 1
+            while (flag_load(fl + f.p0) < u || flag_load(fl + f.p1) < u) {
+                if (wall_clock64() - t0 > CHAIN_WAIT_TICKS) {
+                    expired = true;
+                    break;
+                }
+                __builtin_amdgcn_s_sleep(2);
+            }
+            if (expired) __hip_atomic_store(C.ctl + 1, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+        }
+        run_butterfly<W>(it, u, local, ops, index);
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+        __hip_atomic_store(fl + f.p0, u + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        __hip_atomic_store(fl + f.p1, u + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+}
+
+// Clears the chain's flags of every item and its work counter / error word (ctl[0..1]).
+__global__ void k_chain_reset(Chain C) {
+    if (blockIdx.x == 0 && threadIdx.x < 2) C.ctl[threadIdx.x] = 0;
+    const Item &it = C.L.item[blockIdx.x];
+    const unsigned n = ((it.rows + 63) / 64) << it.logn2;
+    for (unsigned e = threadIdx.x; e < n; e += blockDim.x) it.flags[e] = 0;
+}
+
+// D[pos][row] -> the item's output format.  D is in bit-reversed order, so the natural
+// index of pos is bitrev(pos) and the data-availability order (FK20Py
+// list_to_reverse_bit_order of the natural list) is pos itself.
+//   FK20_OUT_G1P / _DA:               g1p_t, canonical standard projective (144 bytes)
+//   FK20_OUT_COMPRESSED / _DA:        48-byte compressed G1 (g1_compress)
 __global__ void __launch_bounds__(64) k_output_multi(Launch L) {
     unsigned local;
     const Item &it = L.item[find_item(L, blockIdx.x, local)];
@@ -413,9 +523,17 @@ __global__ void __launch_bounds__(64) k_output_multi(Launch L) {
     if (row >= rows) return;
     g1p p;
     pt_load(p, it.D + ((size_t)pos * rows + row) * 36);
-    uint32_t w[36];
-    g1_store_std(w, p);
-    st_words(it.h_fft + ((size_t)row * (1u << logn2) + bitrev(pos, logn2)) * G1P_WORDS, w, 9);
+    const unsigned fmt = it.out_format;
+    const size_t slot = (size_t)row * (1u << logn2) + ((fmt & 1u) ? pos : bitrev(pos, logn2));
+    if (fmt & 2u) {
+        uint32_t w[12];
+        g1_compress(w, p);
+        st_words(it.h_fft + slot * 12, w, 3);
+    } else {
+        uint32_t w[36];
+        g1_store_std(w, p);
+        st_words(it.h_fft + slot * G1P_WORDS, w, 9);
+    }
 }
 
 // ---------------------------------------------------------------- launchers
@@ -445,11 +563,21 @@ static unsigned set_blocks(Launch &L, const unsigned *per_item) {
     return total;
 }
 
+// Resident waves of k_chain: two per SIMD on 256 CUs.
+static constexpr unsigned CHAIN_WAVES = 2048;
+
 hipError_t launch_batch(const Launch &base, const DevTables *T, const uint32_t *ops, const uint32_t *index,
-                        hipStream_t s, LaunchHook hook) {
+                        hipStream_t s, LaunchHook hook, uint32_t *chain) {
     Launch L = base;
     unsigned per[MAX_ITEMS], lmax = 0;
     for (unsigned k = 0; k < L.nitems; k++) lmax = std::max(lmax, L.item[k].logn2);
+    if (chain) {
+        Chain C;
+        memset(&C, 0, sizeof C);
+        C.L = L;
+        C.ctl = chain;
+        hipLaunchKernelGGL(k_chain_reset, dim3(L.nitems), dim3(256), 0, s, C);
+    }
     // Toeplitz + NTT
     for (unsigned k = 0; k < L.nitems; k++) per[k] = L.item[k].rows * 16;
     unsigned total = set_blocks(L, per);
@@ -460,34 +588,64 @@ hipError_t launch_batch(const Launch &base, const DevTables *T, const uint32_t *
     total = set_blocks(L, per);
     if (total) hipLaunchKernelGGL(k_msm_multi, dim3(total), dim3(64), 0, s, L);
     hook(LK_MSM);
-    // IFFT DIT stages, right-aligned to the middle launch
-    for (unsigned t = 0; t + 1 < lmax; t++) {
+    // transform stages: global stage t = 0 .. 2*lmax - 2; item k (L = logn2) runs local stage
+    // u = t - (lmax - L) while 0 <= u <= 2L - 2 (DIT right-aligned to the middle at t = lmax-1,
+    // DIF left-aligned after it)
+    const unsigned nt = 2 * lmax - 1;
+    if (chain) {
+        Chain C;
+        memset(&C, 0, sizeof C);
+        C.L = L;
+        C.ctl = chain;
+        C.nstages = nt;
+        unsigned acc = 0;
+        for (unsigned t = 0; t < nt; t++) {
+            unsigned in_stage = 0;
+            for (unsigned k = 0; k < L.nitems; k++) {
+                const Item &it = L.item[k];
+                const int u = t < lmax - 1 ? (int)t - (int)(lmax - it.logn2)
+                            : (int)it.logn2 - 1 + (int)(t - (lmax - 1));
+                const bool live = t < lmax - 1 ? u >= 0 : u <= (int)(2 * it.logn2 - 2);
+                C.ustage[t][k] = (int8_t)(live ? u : -1);
+                in_stage += live ? (1u << (it.logn2 - 1)) * ((it.rows + 63) / 64) : 0;
+                C.blk_end[t][k] = in_stage;
+            }
+            acc += in_stage;
+            C.t_end[t] = acc;
+        }
+        C.total = acc;
+        if (acc) hipLaunchKernelGGL(k_chain<TW_WNAF>, dim3(std::min(acc, CHAIN_WAVES)), dim3(64), 0, s, C, ops, index);
+        hook(LK_CHAIN);
+    } else {
+        // one launch per stage; IFFT DIT stages right-aligned to the middle launch
+        for (unsigned t = 0; t + 1 < lmax; t++) {
+            for (unsigned k = 0; k < L.nitems; k++) {
+                const int st = (int)t - (int)(lmax - L.item[k].logn2);
+                L.item[k].stage = st;
+                per[k] = st >= 0 ? (1u << (L.item[k].logn2 - 1)) * ((L.item[k].rows + 63) / 64) : 0;
+            }
+            total = set_blocks(L, per);
+            if (total) hipLaunchKernelGGL(k_dit_multi<TW_WNAF>, dim3(total), dim3(64), 0, s, L, ops, index);
+            hook(LK_DIT);
+        }
         for (unsigned k = 0; k < L.nitems; k++) {
-            const int st = (int)t - (int)(lmax - L.item[k].logn2);
-            L.item[k].stage = st;
-            per[k] = st >= 0 ? (1u << (L.item[k].logn2 - 1)) * ((L.item[k].rows + 63) / 64) : 0;
+            L.item[k].stage = 0;
+            per[k] = (1u << (L.item[k].logn2 - 1)) * ((L.item[k].rows + 63) / 64);
         }
         total = set_blocks(L, per);
-        if (total) hipLaunchKernelGGL(k_dit_multi<TW_WNAF>, dim3(total), dim3(64), 0, s, L, ops, index);
-        hook(LK_DIT);
-    }
-    for (unsigned k = 0; k < L.nitems; k++) {
-        L.item[k].stage = 0;
-        per[k] = (1u << (L.item[k].logn2 - 1)) * ((L.item[k].rows + 63) / 64);
-    }
-    total = set_blocks(L, per);
-    if (total) hipLaunchKernelGGL(k_middle_multi<TW_WNAF>, dim3(total), dim3(64), 0, s, L, ops, index);
-    hook(LK_MIDDLE);
-    // FFT DIF stages 1.., left-aligned after the middle launch
-    for (unsigned t = 1; t < lmax; t++) {
-        for (unsigned k = 0; k < L.nitems; k++) {
-            const int st = t < L.item[k].logn2 ? (int)t : -1;
-            L.item[k].stage = st;
-            per[k] = st >= 0 ? (1u << (L.item[k].logn2 - 1)) * ((L.item[k].rows + 63) / 64) : 0;
+        if (total) hipLaunchKernelGGL(k_middle_multi<TW_WNAF>, dim3(total), dim3(64), 0, s, L, ops, index);
+        hook(LK_MIDDLE);
+        // FFT DIF stages 1.., left-aligned after the middle launch
+        for (unsigned t = 1; t < lmax; t++) {
+            for (unsigned k = 0; k < L.nitems; k++) {
+                const int st = t < L.item[k].logn2 ? (int)t : -1;
+                L.item[k].stage = st;
+                per[k] = st >= 0 ? (1u << (L.item[k].logn2 - 1)) * ((L.item[k].rows + 63) / 64) : 0;
+            }
+            total = set_blocks(L, per);
+            if (total) hipLaunchKernelGGL(k_dif_multi<TW_WNAF>, dim3(total), dim3(64), 0, s, L, ops, index);
+            hook(LK_DIF);
         }
-        total = set_blocks(L, per);
-        if (total) hipLaunchKernelGGL(k_dif_multi<TW_WNAF>, dim3(total), dim3(64), 0, s, L, ops, index);
-        hook(LK_DIF);
     }
     // output
     for (unsigned k = 0; k < L.nitems; k++) per[k] = (1u << L.item[k].logn2) * ((L.item[k].rows + 63) / 64);

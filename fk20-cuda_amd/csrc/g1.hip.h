@@ -203,4 +203,30 @@ FK20_HD bool g1_to_affine(fe &ax, fe &ay, const g1p &p) {
     return false;
 }
 
+// 48-byte compressed serialisation, as 12 words in memory order: big-endian canonical x, top
+// byte flags 0x80 (compressed), 0x40 (infinity: 0xc0 then zeros), 0x20 (y is the larger of
+// y and p - y).
+FK20_HD void g1_compress(uint32_t *w, const g1p &p) {
+    fe x, y;
+    if (g1_to_affine(x, y, p)) {
+#pragma unroll
+        for (int i = 0; i < 12; i++) w[i] = 0;
+        w[0] = 0xc0u;
+        return;
+    }
+    fe_from_mont(x, x);
+    fe_from_mont(y, y);
+    uint32_t xw[12], yw[12];
+    fe_pack(xw, x);
+    fe_pack(yw, y);
+    // y > (p - 1) / 2, compared from the top word
+    int cmp = 0;
+#pragma unroll
+    for (int i = 11; i >= 0; i--)
+        if (cmp == 0) cmp = yw[i] > FP_HALF_PACKED[i] ? 1 : yw[i] < FP_HALF_PACKED[i] ? -1 : 0;
+#pragma unroll
+    for (int k = 0; k < 12; k++) w[k] = __builtin_bswap32(xw[11 - k]);
+    w[0] |= 0x80u | (cmp > 0 ? 0x20u : 0u);
+}
+
 }  // namespace fk20

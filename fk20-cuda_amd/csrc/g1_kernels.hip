@@ -246,6 +246,18 @@ __global__ void k_to_affine(uint32_t *out, const uint32_t *in, size_t count) {
     for (int i = 0; i < 6; i++) q[i] = make_uint4(w[4 * i], w[4 * i + 1], w[4 * i + 2], w[4 * i + 3]);
 }
 
+__global__ void k_compress(uint32_t *out, const uint32_t *in, size_t count) {
+    const size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= count) return;
+    g1p p;
+    g1_gload_std(p, in + e * G1P_WORDS);
+    uint32_t w[12];
+    g1_compress(w, p);
+    uint4 *q = reinterpret_cast<uint4 *>(out + e * 12);
+#pragma unroll
+    for (int i = 0; i < 3; i++) q[i] = make_uint4(w[4 * i], w[4 * i + 1], w[4 * i + 2], w[4 * i + 3]);
+}
+
 // ---------------------------------------------------------------- element-wise ops
 
 __device__ __forceinline__ void fe_gload_std(fe &a, const uint32_t *g) {
@@ -425,6 +437,12 @@ hipError_t launch_setup_gather(uint32_t *xext, const uint32_t *srs, unsigned N, 
 hipError_t launch_to_affine(uint32_t *out, const uint32_t *in, size_t count, hipStream_t s) {
     if (!count) return hipSuccess;
     hipLaunchKernelGGL(k_to_affine, dim3((unsigned)((count + 63) / 64)), dim3(64), 0, s, out, in, count);
+    return hipGetLastError();
+}
+
+hipError_t launch_compress(uint32_t *out, const uint32_t *in, size_t count, hipStream_t s) {
+    if (!count) return hipSuccess;
+    hipLaunchKernelGGL(k_compress, dim3((unsigned)((count + 63) / 64)), dim3(64), 0, s, out, in, count);
     return hipGetLastError();
 }
 

@@ -4,7 +4,7 @@
 // `nslots` batches in flight, one per stream.  Whenever a slot is free it takes pending
 // requests in arrival order while they fit the slot's share of the device-memory budget
 // (per row: toeplitz_fft 16*n2 Fr + hext n2 G1 workspace, plus the polynomial in and h_fft
-// out), packs them per setup into one fk20_prove_batch_ws pass (copy in, prove, copy out on
+// out in the context's output format, fixed when the server is created), packs them per setup into one fk20_prove_batch_ws pass (copy in, prove, copy out on
 // the slot's stream) and records an event.  Completed slots are reaped by event query, so
 // requests that arrive while batches run are admitted as soon as a slot frees: batches on
 // different streams overlap on the device.
@@ -50,9 +50,10 @@ struct Slot {
     std::vector<uint64_t> reqs;
 };
 
-size_t row_bytes(unsigned N) {
+// device bytes of one row: workspace, polynomial in, h_fft out (in the context's format)
+size_t row_bytes(unsigned N, size_t point_bytes) {
     const size_t n2 = N / 8;
-    return fk20_rows_workspace_bytes(1, N) + (size_t)N * 32 + n2 * 144;
+    return fk20_rows_workspace_bytes(1, N) + (size_t)N * 32 + n2 * point_bytes;
 }
 
 }  // namespace
@@ -62,6 +63,7 @@ struct fk20_server {
     int device = 0;
     size_t budget = 0;
     unsigned max_batch_rows = 0;
+    size_t point_bytes = 144;  // output bytes per point, from the context's format at creation
     std::vector<Slot> slots;
     std::mutex mu;
     std::condition_variable cv_work, cv_done;
@@ -78,6 +80,10 @@ struct fk20_server {
 };
 
 int fk20_server::launch(Slot &s, std::vector<uint64_t> &ids) {
+    // output buffers were sized for the format fixed at creation
+    int format = FK20_OUT_G1P;
+    if (fk20_ctx_get_output(ctx, &format) != FK20_OK || fk20_output_point_bytes(format) != point_bytes)
+        return FK20_EINVAL;
     // group the requests by setup: one batch item per setup, rows packed contiguously
     std::map<int, std::vector<uint64_t>> by_setup;
     for (uint64_t id : ids) by_setup[reqs[id].setup].push_back(id);
@@ -105,7 +111,7 @@ int fk20_server::launch(Slot &s, std::vector<uint64_t> &ids) {
         it.h_fft = reinterpret_cast<fk20_g1p_t *>(p);
         for (uint64_t id : kv.second) {
             const Request &r = reqs[id];
-            const size_t b = (size_t)r.rows * (r.N / 8) * 144;
+            const size_t b = (size_t)r.rows * (r.N / 8) * point_bytes;
             outs.push_back({r.out, p, b});
             p += b;
         }
@@ -158,7 +164,7 @@ void fk20_server::run() {
             std::map<int, int> setups;
             while (!pending.empty()) {
                 const Request &r = reqs[pending.front()];
-                const size_t b = (size_t)r.rows * row_bytes(r.N);
+                const size_t b = (size_t)r.rows * row_bytes(r.N, point_bytes);
                 const bool new_setup = !setups.count(r.setup);
                 if (!take.empty() && (bytes + b > s.bytes || rows + r.rows > max_batch_rows ||
                                       (new_setup && setups.size() >= 16)))
@@ -205,10 +211,14 @@ extern "C" {
 int fk20_server_create(fk20_server_t **out, fk20_ctx_t *ctx, size_t budget_bytes, unsigned nslots,
                        unsigned max_batch_rows) {
     if (!out || !ctx || !nslots || nslots > 8 || !max_batch_rows) return FK20_EINVAL;
+    int format = FK20_OUT_G1P;
+    if (fk20_ctx_get_output(ctx, &format) != FK20_OK) return FK20_EINVAL;
+    const size_t point_bytes = fk20_output_point_bytes(format);
     // each slot must hold at least one N=4096 row
-    if (budget_bytes / nslots < row_bytes(4096)) return FK20_EINVAL;
+    if (budget_bytes / nslots < row_bytes(4096, point_bytes)) return FK20_EINVAL;
     fk20_server *srv = new fk20_server();
     srv->ctx = ctx;
+    srv->point_bytes = point_bytes;
     srv->budget = budget_bytes;
     srv->max_batch_rows = max_batch_rows;
     hipError_t e = hipGetDevice(&srv->device);
@@ -240,7 +250,7 @@ int fk20_server_submit(fk20_server_t *srv, int setup_id, const fk20_fr_t *host_p
     unsigned N = 0, win = 0;
     int rc = fk20_setup_info(srv->ctx, setup_id, &N, &win);
     if (rc != FK20_OK) return rc;
-    if ((size_t)rows * row_bytes(N) > srv->budget / srv->slots.size() || rows > srv->max_batch_rows)
+    if ((size_t)rows * row_bytes(N, srv->point_bytes) > srv->budget / srv->slots.size() || rows > srv->max_batch_rows)
         return FK20_ENOMEM;  // larger than one slot's share of the budget
     std::lock_guard<std::mutex> lk(srv->mu);
     if (srv->stop) return FK20_EINVAL;

@@ -1,12 +1,12 @@
 """Autotuner for the batched path: picks the MSM window width per polynomial size.
 
-For each N it builds the setup tables of every candidate window (8, 10, 12 bits; larger
-windows mean fewer mixed additions per term but 4x the table memory per 2 bits), times
+For each N it builds the setup tables of every candidate window (8, 10, 12, 13 bits; larger
+windows mean fewer mixed additions per term but twice the table memory per bit), times
 fk20_prove_rows over a batch with HIP events, and keeps the fastest whose tables fit the
 memory cap.  Results persist as JSON keyed by device name and N (tuned.json next to the
 library), and bench.py / callers read them with `tuned_window`.
 
-    python3 -m fk20_amd.autotune [--rows 512] [--sizes 1024,2048,4096] [--max-table-gb 40]
+    python3 -m fk20_amd.autotune [--rows 512] [--sizes 1024,2048,4096] [--max-table-gb 72]
 """
 import argparse
 import json
@@ -21,7 +21,7 @@ from . import synthetic as S
 from .prover import Prover, setup_table_bytes
 
 TUNED = os.path.join(L.PKG_DIR, "tuned.json")
-CANDIDATE_WINDOWS = (8, 10, 12)
+CANDIDATE_WINDOWS = (8, 10, 12, 13)
 
 
 def device_name():
@@ -60,7 +60,7 @@ def _time_window(prover, xext_dev, N, window, poly_dev, out_dev, rows, reps):
     return statistics.median(ms)
 
 
-def tune(sizes=(1024, 2048, 4096), rows=512, max_table_gb=40.0, reps=3, seed=11, xext_for=None):
+def tune(sizes=(1024, 2048, 4096), rows=512, max_table_gb=72.0, reps=3, seed=11, xext_for=None):
     """xext_for(N) -> DeviceBuffer of xext_fft[16][N/8] (a synthetic setup by default)."""
     if xext_for is None:
         sys.path.insert(0, os.path.dirname(L.PKG_DIR))
@@ -93,7 +93,7 @@ def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--rows", type=int, default=512)
     ap.add_argument("--sizes", default="1024,2048,4096")
-    ap.add_argument("--max-table-gb", type=float, default=40.0)
+    ap.add_argument("--max-table-gb", type=float, default=72.0)
     args = ap.parse_args()
     res = tune(tuple(int(x) for x in args.sizes.split(",")), args.rows, args.max_table_gb)
     print(json.dumps(res))

@@ -51,6 +51,11 @@ _SIGS = {
     "fk20_ctx_create": ([ctypes.POINTER(_P)], _I),
     "fk20_ctx_destroy": ([_P], _I),
     "fk20_ctx_set_graph": ([_P, _I], _I),
+    "fk20_ctx_set_chain": ([_P, _I], _I),
+    "fk20_ctx_set_output": ([_P, _I], _I),
+    "fk20_ctx_get_output": ([_P, ctypes.POINTER(_I)], _I),
+    "fk20_output_point_bytes": ([_I], _SZ),
+    "fk20_g1p_compress": ([_P, _P, _SZ, _P], _I),
     "fk20_setup_table_bytes": ([_U, _U], _SZ),
     "fk20_setup_load": ([_P, _P, _U, _U, ctypes.POINTER(_I), _P], _I),
     "fk20_setup_unload": ([_P, _I], _I),
@@ -68,7 +73,7 @@ _SIGS = {
     "fk20_server_destroy": ([_P], _I),
     "fk20_host_alloc": ([ctypes.POINTER(_P), _SZ], _I),
     "fk20_host_free": ([_P], _I),
-    "fk20_twiddle_selfcheck": ([ctypes.POINTER(_U)], _I),
+    "fk20_twiddle_selfcheck": ([ctypes.POINTER(_U), ctypes.POINTER(_U)], _I),
     "fk20_ctx_set_launch_timing": ([_P, _I], _I),
     "fk20_ctx_launch_times": ([_P, ctypes.POINTER(ctypes.c_float), ctypes.POINTER(_I), _I, ctypes.POINTER(_I)], _I),
     "fk20_status_string": ([_I], ctypes.c_char_p),

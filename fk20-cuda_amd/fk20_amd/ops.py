@@ -87,6 +87,20 @@ def to_affine(pts):
     return _run("fk20_g1p_to_affine", pts.shape[:-1] + (12,), [pts], [count])
 
 
+def compress(pts):
+    """48-byte compressed serialisation of g1p points (..., 18) -> uint8 (..., 48)."""
+    pts = _u64(pts)
+    count = pts.size // 18
+    buf = DeviceBuffer.from_array(pts)
+    out = DeviceBuffer(max(1, count) * 48)
+    call("fk20_g1p_compress", out.ptr, buf.ptr, count, None)
+    synchronize()
+    res = out.download(pts.shape[:-1] + (48,), np.uint8)
+    buf.free()
+    out.free()
+    return res
+
+
 FP_OPS = {"add": 0, "sub": 1, "mul": 2, "sqr": 3, "neg": 4, "inv": 5, "reduce": 6}
 FR_OPS = FP_OPS
 G1_OPS = {"add": 0, "sub": 1, "dbl": 2, "mul": 3, "neg": 4, "add_affine": 5, "is_point": 6, "eq": 7}

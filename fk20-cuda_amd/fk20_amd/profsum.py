@@ -3,6 +3,10 @@
 `stats`: the kernel-stats CSV of a `rocprofv3 --kernel-trace --stats -f csv` run as a
 Markdown table, with the command that made it.
 
+`valu`: executed VALU instructions per kernel launch from a counter pass over SQ_INSTS_VALU,
+SQ_INSTS_VALU_INT32/INT64, SQ_ACTIVE_INST_VALU, SQ_BUSY_CYCLES and GRBM_GUI_ACTIVE: the
+integer instruction mix, the issue rate and the VALU-busy fraction of each kernel.
+
 `traffic`: HBM traffic per kernel launch from two counter passes (FETCH_SIZE, WRITE_SIZE).
 Each pass is a `rocprofv3 --pmc <counter> -f csv` run of `bench.py --calibrate-copy-mb M`,
 which copies M MiB device to device before the timed work.  The counters' readings for that
@@ -60,6 +64,46 @@ def summarize(fetch, write, copy_mb):
             "kernels": kernels}
 
 
+VALU_COUNTERS = ("SQ_INSTS_VALU", "SQ_INSTS_VALU_INT32", "SQ_INSTS_VALU_INT64", "SQ_ACTIVE_INST_VALU",
+                 "SQ_INSTS_SALU", "SQ_WAVES", "SQ_BUSY_CYCLES", "GRBM_GUI_ACTIVE")
+SIMDS = 1024  # 256 CUs x 4 SIMDs
+XCDS = 8      # GRBM_GUI_ACTIVE is summed over the 8 XCDs
+
+
+def valu_summary(counter_csv, trace_csv):
+    """Per kernel: mean counters per dispatch, mean duration (from a kernel trace of the same
+    workload), lane-instruction rate and VALU-busy fraction."""
+    cnt = collections.defaultdict(lambda: collections.defaultdict(list))
+    with open(counter_csv) as f:
+        for r in csv.DictReader(f):
+            cnt[_kernel(r["Kernel_Name"])][r["Counter_Name"]].append(float(r["Counter_Value"]))
+    dur = collections.defaultdict(list)
+    with open(trace_csv) as f:
+        for r in csv.DictReader(f):
+            dur[_kernel(r["Kernel_Name"])].append((int(r["End_Timestamp"]) - int(r["Start_Timestamp"])) * 1e-9)
+    out = {}
+    for k, c in cnt.items():
+        if k.startswith("__amd_rocclr") or "SQ_INSTS_VALU" not in c:
+            continue
+        m = {n: sum(v) / len(v) for n, v in c.items()}
+        t = sum(dur[k]) / len(dur[k]) if dur.get(k) else None
+        cycles = m.get("GRBM_GUI_ACTIVE", 0) / XCDS
+        e = {"dispatches": len(c["SQ_INSTS_VALU"]),
+             "valu_wave_instr": m["SQ_INSTS_VALU"],
+             "int64_share": m.get("SQ_INSTS_VALU_INT64", 0) / m["SQ_INSTS_VALU"],
+             "int32_share": m.get("SQ_INSTS_VALU_INT32", 0) / m["SQ_INSTS_VALU"],
+             "salu_instr": m.get("SQ_INSTS_SALU", 0), "waves": m.get("SQ_WAVES", 0),
+             "gpu_cycles": cycles,
+             # SQ_ACTIVE_INST_VALU counts quad-cycles: one per wave64 instruction on a SIMD16
+             "valu_busy": 4 * m.get("SQ_ACTIVE_INST_VALU", 0) / (SIMDS * cycles) if cycles else None}
+        if t:
+            e["avg_s"] = t
+            e["clock_ghz"] = cycles / t / 1e9 if cycles else None
+            e["lane_instr_per_s"] = m["SQ_INSTS_VALU"] * 64 / t
+        out[k] = e
+    return out
+
+
 def stats_md(csv_path, title, command, note=""):
     rows = list(csv.DictReader(open(csv_path)))
     lines = ["# " + title, "", "Command (from the repository root on one MI355X):", "", "```", command.strip(), "```", ""]
@@ -84,6 +128,13 @@ def main():
     t.add_argument("--json", required=True)
     t.add_argument("--md", required=True)
     t.add_argument("--command", default="", help="the rocprofv3 commands, for the record")
+    v = sub.add_parser("valu")
+    v.add_argument("--counters", required=True)
+    v.add_argument("--trace", required=True, help="kernel_trace.csv of a run of the same workload")
+    v.add_argument("--config", required=True)
+    v.add_argument("--json", required=True)
+    v.add_argument("--md", required=True)
+    v.add_argument("--command", default="")
     k = sub.add_parser("stats")
     k.add_argument("--csv", required=True)
     k.add_argument("--title", required=True)
@@ -91,6 +142,29 @@ def main():
     k.add_argument("--note", default="")
     k.add_argument("--md", required=True)
     args = ap.parse_args()
+    if args.cmd == "valu":
+        kern = valu_summary(args.counters, args.trace)
+        cfg = {a: int(b) for a, b in (x.split("=") for x in args.config.split(","))}
+        with open(args.json, "w") as f:
+            json.dump({"config": cfg, "command": args.command, "kernels": kern}, f, indent=1, sort_keys=True)
+        lines = ["# Executed VALU instructions per launch (rocprofv3 SQ counters)", "",
+                 "Workload: %s.  Counters are summed over all SEs; GRBM_GUI_ACTIVE over the 8 XCDs "
+                 "(cycles = value / 8); SQ_ACTIVE_INST_VALU is in quad-cycles, so VALU busy = "
+                 "4 x SQ_ACTIVE_INST_VALU / (1024 SIMDs x cycles).  Durations come from the kernel "
+                 "trace of the same workload." % ", ".join("%s=%d" % kv for kv in sorted(cfg.items())), "",
+                 "| kernel | dispatches | VALU wave-instr / launch | INT64 share | INT32 share | avg ms | "
+                 "T lane-instr/s | clock GHz | VALU busy |", "|---|---:|---:|---:|---:|---:|---:|---:|---:|"]
+        for name, e in sorted(kern.items(), key=lambda kv: -kv[1]["valu_wave_instr"] * kv[1]["dispatches"]):
+            lines.append("| `%s` | %d | %.4g | %.3f | %.3f | %s | %s | %s | %s |" % (
+                name, e["dispatches"], e["valu_wave_instr"], e["int64_share"], e["int32_share"],
+                "%.3f" % (e["avg_s"] * 1e3) if e.get("avg_s") else "-",
+                "%.2f" % (e["lane_instr_per_s"] / 1e12) if e.get("lane_instr_per_s") else "-",
+                "%.2f" % e["clock_ghz"] if e.get("clock_ghz") else "-",
+                "%.3f" % e["valu_busy"] if e.get("valu_busy") is not None else "-"))
+        lines += ["", "Commands:", "", "```", args.command.strip(), "```", ""]
+        with open(args.md, "w") as f:
+            f.write("\n".join(lines))
+        return
     if args.cmd == "stats":
         with open(args.md, "w") as f:
             f.write(stats_md(args.csv, args.title, args.command, args.note))

@@ -61,22 +61,35 @@ class Prover:
             arr[k].h_fft = out.ptr if isinstance(out, DeviceBuffer) else out
         call("fk20_prove_batch", self.ctx, arr, len(items), stream)
 
+    def set_output(self, fmt):
+        """Output format of the proofs: one of OUTPUT_FORMATS (default "g1p")."""
+        call("fk20_ctx_set_output", self.ctx, OUTPUT_FORMATS[fmt])
+        self.out_format = fmt
+
+    def _out_shape(self, rows, N):
+        fmt = getattr(self, "out_format", "g1p")
+        if fmt.startswith("compressed"):
+            return (rows, N // 8, 48), np.uint8
+        return (rows, N // 8, 18), np.uint64
+
     def prove_batch(self, groups):
-        """groups: [(setup_id, host polys (rows, N, 4))] -> [host h_fft (rows, N/8, 18)]."""
+        """groups: [(setup_id, host polys (rows, N, 4))] -> [host h_fft (rows, N/8, 18)], or
+        (rows, N/8, 48) uint8 in the compressed output formats."""
         bufs, items = [], []
         for sid, poly in groups:
             poly = np.ascontiguousarray(poly, dtype=np.uint64)
             rows, N = poly.shape[0], poly.shape[1]
             assert self.setups[sid] == N
             pb = DeviceBuffer.from_array(poly)
-            hb = DeviceBuffer(rows * (N // 8) * 144)
-            bufs.append((pb, hb, rows, N))
+            shape, dt = self._out_shape(rows, N)
+            hb = DeviceBuffer(int(np.prod(shape)) * np.dtype(dt).itemsize)
+            bufs.append((pb, hb, shape, dt))
             items.append((sid, rows, pb, hb))
         self.prove_batch_device(items)
         synchronize()
         outs = []
-        for pb, hb, rows, N in bufs:
-            outs.append(hb.download((rows, N // 8, 18)))
+        for pb, hb, shape, dt in bufs:
+            outs.append(hb.download(shape, dt))
             pb.free()
             hb.free()
         return outs
@@ -87,13 +100,18 @@ class Prover:
         rows, N = poly.shape[0], poly.shape[1]
         assert self.setups[sid] == N
         pb = DeviceBuffer.from_array(poly)
-        hb = DeviceBuffer(rows * (N // 8) * 144)
+        shape, dt = self._out_shape(rows, N)
+        hb = DeviceBuffer(int(np.prod(shape)) * np.dtype(dt).itemsize)
         self.prove_device(sid, hb, pb, rows)
         synchronize()
-        out = hb.download((rows, N // 8, 18))
+        out = hb.download(shape, dt)
         pb.free()
         hb.free()
         return out
+
+    def set_chain(self, enable):
+        """Transform stages as one persistent launch, or one launch per stage (default)."""
+        call("fk20_ctx_set_chain", self.ctx, int(bool(enable)))
 
     def set_graph(self, enable):
         call("fk20_ctx_set_graph", self.ctx, int(bool(enable)))
@@ -126,10 +144,14 @@ def setup_table_bytes(N, window):
 
 
 def twiddle_selfcheck():
-    n = ctypes.c_uint()
-    status = lib().fk20_twiddle_selfcheck(ctypes.byref(n))
-    return status, n.value
+    """(status, program size in ops, wNAF digit width) of the library's twiddle programs."""
+    n, w = ctypes.c_uint(), ctypes.c_uint()
+    status = lib().fk20_twiddle_selfcheck(ctypes.byref(n), ctypes.byref(w))
+    return status, n.value, w.value
 
+
+# fk20_ctx_set_output formats (include/fk20.h FK20_OUT_*)
+OUTPUT_FORMATS = {"g1p": 0, "g1p_da": 1, "compressed": 2, "compressed_da": 3}
 
 LAUNCH_KINDS = {0: "k_toeplitz_fft_T", 1: "k_msm_comb", 2: "k_dit_stage", 3: "k_middle", 4: "k_dif_stage",
-                5: "k_fft_output"}
+                5: "k_fft_output", 6: "k_transform_chain"}

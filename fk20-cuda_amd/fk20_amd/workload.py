@@ -61,7 +61,8 @@ def launch_mul32(N, kind, rows):
     """Canonical work of one launch of the batched path over `rows` rows.
 
     k_msm_comb: the MSM; k_dit_stage / k_dif_stage: one radix-2 stage of (n2/2) butterflies;
-    k_middle: the last IFFT and first FFT stages (two stages).  The IFFT's n2 scaling
+    k_middle: the last IFFT and first FFT stages (two stages); k_transform_chain: all of
+    them (2 log2(n2) stages).  The IFFT's n2 scaling
     multiplications (folded into the setup tables here) are not attributed to any launch.
     """
     n2 = N // 8
@@ -71,6 +72,7 @@ def launch_mul32(N, kind, rows):
         "k_dit_stage": stage,
         "k_dif_stage": stage,
         "k_middle": 2 * stage,
+        "k_transform_chain": 2 * int(math.log2(n2)) * stage,
         "k_toeplitz_fft_T": 16 * (n2 // 2) * int(math.log2(n2)) * FR_MUL32,
         "k_fft_output": 0,
     }[kind]
@@ -80,8 +82,8 @@ def launch_mul32(N, kind, rows):
 #
 # Fp multiplications the batched path's algorithms actually perform (same 300-mul32
 # convention), for the roofline of its kernels:
-#   twiddle w^e: GLV split w^e = k2*lambda + k1, wNAF-4 digits of k1 and k2; table
-#     P, 3P, 5P, 7P (1 dbl + 3 add); one dbl per digit position after the first, one add
+#   twiddle w^e: GLV split w^e = k2*lambda + k1, wNAF-5 digits of k1 and k2; table
+#     P, 3P, .., 15P (1 dbl + 7 add); one dbl per digit position after the first, one add
 #     per non-zero digit after the first, one Fp mul per phi-digit
 #   butterfly: twiddle (if e != 0) + 2 complete additions
 #   MSM: one mixed addition (11 Fp mul) per non-zero signed window digit, 16 terms per column
@@ -89,7 +91,7 @@ def launch_mul32(N, kind, rows):
 R_MOD = 0x73EDA753299D7D483339D80809A1D80553BDA402FFFE5BFEFFFFFFFF00000001
 GLV_LAMBDA = 0xAC45A4010001A40200000000FFFFFFFF
 MIXED_ADD = 11
-WNAF_W = 4
+WNAF_W = 5  # the library reports its width (fk20_twiddle_selfcheck); a test ties the two
 
 
 def _wnaf(k, w):
@@ -132,7 +134,7 @@ def twiddle_program(e_idx, inverse=False):
 
 def twiddle_fp_muls(e_idx, inverse=False):
     dbl, adds, phi, _ = twiddle_program(e_idx, inverse)
-    table = DBL + 3 * ADD
+    table = DBL + ((1 << (WNAF_W - 2)) - 1) * ADD
     return table + dbl * DBL + adds * ADD + phi
 
 
@@ -155,6 +157,10 @@ def executed_launch_fp_muls(N, kind, stage=None, window=8):
                 idx = j << (8 - lh)
             total += 2 * ADD + (twiddle_fp_muls(idx, inverse=(kind == "k_dit_stage")) if idx else 0)
         return total
+    if kind == "k_transform_chain":
+        return (sum(executed_launch_fp_muls(N, "k_dit_stage", s) for s in range(lg - 1))
+                + executed_launch_fp_muls(N, "k_middle")
+                + sum(executed_launch_fp_muls(N, "k_dif_stage", s) for s in range(1, lg)))
     if kind == "k_middle":
         total = 0
         for j in range(n2 // 2):

@@ -121,15 +121,37 @@ int fk20_ctx_destroy(fk20_ctx_t *ctx);
 /* 1: fk20_prove_rows captures its launch sequence into a hipGraph per (setup, rows,
  * buffers, stream) and replays it (needs a non-NULL stream); 0: direct launches. */
 int fk20_ctx_set_graph(fk20_ctx_t *ctx, int enable);
+/* 1: the G1 IFFT / zero / FFT stages of a batch run as one persistent launch in which each
+ * 64-row butterfly starts as soon as the two butterflies it reads have finished; 0 (default):
+ * one launch per stage.  Results are identical; on MI355X the per-stage launches are faster
+ * (the persistent form pays agent-scope cache write-back/invalidate around every butterfly). */
+int fk20_ctx_set_chain(fk20_ctx_t *ctx, int enable);
+/* Output format of fk20_prove_rows / fk20_prove_batch (h_fft buffers), per context:
+ *   FK20_OUT_G1P           g1p_t[rows][n2], natural order: the reference ABI (default)
+ *   FK20_OUT_G1P_DA        g1p_t, data-availability order: FK20Py list_to_reverse_bit_order
+ *                          of the natural list (fk20_multi.py:107-118, kzg_proofs.py:107-118)
+ *   FK20_OUT_COMPRESSED    48-byte compressed G1 per point (big-endian x; flags 0x80
+ *                          compressed, 0x40 infinity, 0x20 larger y), natural order
+ *   FK20_OUT_COMPRESSED_DA the same in data-availability order
+ * No reference counterpart: the reference emits g1p_t only. */
+#define FK20_OUT_G1P 0
+#define FK20_OUT_G1P_DA 1
+#define FK20_OUT_COMPRESSED 2
+#define FK20_OUT_COMPRESSED_DA 3
+int fk20_ctx_set_output(fk20_ctx_t *ctx, int format);
+int fk20_ctx_get_output(fk20_ctx_t *ctx, int *format);
+/* Bytes per output point of a format (144 or 48; 0 for an unknown format). */
+size_t fk20_output_point_bytes(int format);
 /* Device bytes of one setup's window tables for polynomial length N and window width w
- * (8, 10 or 12 bits): 16 * (N/8) * ceil(256/w) * 2^(w-1) * 96. */
+ * (8 to 13 bits): 16 * (N/8) * ceil(256/w) * 2^(w-1) * 96, e.g. 35 GiB at N=4096, w=12. */
 size_t fk20_setup_table_bytes(unsigned N, unsigned window);
 /* Builds the resident tables of xext_fft[16][N/8] (device memory, standard form);
  * synchronises `stream`.  The returned id names the setup in fk20_prove_rows. */
 int fk20_setup_load(fk20_ctx_t *ctx, const fk20_g1p_t *xext_fft, unsigned N, unsigned window,
                     int *setup_id, fk20_stream_t stream);
 int fk20_setup_unload(fk20_ctx_t *ctx, int setup_id);
-/* Workspace bytes fk20_prove_rows needs for `rows` rows of length N. */
+/* Workspace bytes fk20_prove_rows needs for `rows` rows of length N:
+ * rows * (N/8) * 656 + ceil(rows/64) * (N/8) * 4 + 256. */
 size_t fk20_rows_workspace_bytes(unsigned rows, unsigned N);
 /* Grows the context workspace to fit `rows` rows of length N. */
 int fk20_ctx_reserve(fk20_ctx_t *ctx, unsigned rows, unsigned N);
@@ -166,7 +188,9 @@ int fk20_setup_info(fk20_ctx_t *ctx, int setup_id, unsigned *N, unsigned *window
  * (N/8 G1) workspace plus the polynomial and h_fft -- and at most max_batch_rows rows and
  * 16 setups, packs them per setup into one batched pass, and copies results back.
  * Requests arriving while batches run are admitted as slots free up; results are the same
- * as proving each request alone. */
+ * as proving each request alone.  Outputs use the context's output format as it is when the
+ * server is created (host_h_fft then holds rows * N/8 points of that format); batches
+ * launched after a format change fail with FK20_EINVAL. */
 typedef struct fk20_server fk20_server_t;
 int fk20_server_create(fk20_server_t **server, fk20_ctx_t *ctx, size_t budget_bytes, unsigned nslots,
                        unsigned max_batch_rows);
@@ -190,13 +214,17 @@ int fk20_host_free(void *ptr);
 int fk20_ctx_set_launch_timing(fk20_ctx_t *ctx, int enable);
 int fk20_ctx_launch_times(fk20_ctx_t *ctx, float *ms, int *kinds, int max, int *count);
 /* Host-only check of the twiddle programs (GLV split + wNAF of every w^+-i, i < 512,
- * evaluated over Fr); total_ops receives the program size.  Needs no device. */
-int fk20_twiddle_selfcheck(unsigned *total_ops);
+ * evaluated over Fr); total_ops receives the program size and wnaf_width the digit width
+ * the programs (and the transform kernels' tables of odd multiples) use.  Needs no device. */
+int fk20_twiddle_selfcheck(unsigned *total_ops, unsigned *wnaf_width);
 
 /* ------------------------------------------------------------------ helpers */
 
 /* Canonical affine form of count points.  Replaces g1a_fromG1p_wrapper (g1.cuh:98; g1p.cu:168). */
 int fk20_g1p_to_affine(fk20_g1a_t *out, const fk20_g1p_t *in, size_t count, fk20_stream_t stream);
+/* 48-byte compressed serialisation of count points (out: count * 48 bytes), the format of
+ * FK20_OUT_COMPRESSED.  No reference counterpart (the reference stores g1p_t, toByte.py:41). */
+int fk20_g1p_compress(uint8_t *out, const fk20_g1p_t *in, size_t count, fk20_stream_t stream);
 
 /* Element-wise arithmetic over count elements, the counterparts of the reference's test
  * wrappers (g1test_fft.cu:39 g1p_add_wrapper, :70 g1p_mul_wrapper; fptest/frtest kernels).

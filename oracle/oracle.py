@@ -238,3 +238,49 @@ def kzg_quotient_at(poly, c, tau):
 
 def default_threads():
     return int(lib().or_default_threads())
+
+
+# ---- output formats ----
+
+def reverse_bit_order(x):
+    """x[..., bitrev(i), ...] along axis 0 of the point index: FK20Py list_to_reverse_bit_order
+    (kzg_proofs.py:107-118) applied to the last point axis of x (shape (..., n, k))."""
+    x = np.asarray(x)
+    n = x.shape[-2]
+    bits = n.bit_length() - 1
+    idx = [int(format(i, "0%db" % bits)[::-1], 2) for i in range(n)]
+    return x[..., idx, :]
+
+
+def g1_compress(p):
+    """48-byte compressed serialisation of g1p points (shape (..., 18)) -> uint8 (..., 48):
+    big-endian canonical x; top byte flags 0x80 compressed, 0x40 infinity, 0x20 set when y is
+    the larger of (y, p - y).  Infinity is 0xc0 followed by zeros."""
+    p = np.ascontiguousarray(p, dtype=np.uint64)
+    flat = p.reshape(-1, 18)
+    aff = to_affine(flat)
+    out = np.zeros((flat.shape[0], 48), dtype=np.uint8)
+    for k in range(flat.shape[0]):
+        if g1p_is_inf(flat[k]):
+            out[k, 0] = 0xC0
+            continue
+        x, y = limbs_to_int(aff[k, :6]), limbs_to_int(aff[k, 6:])
+        b = bytearray(x.to_bytes(48, "big"))
+        b[0] |= 0x80 | (0x20 if y > P_MOD - y else 0)
+        out[k] = np.frombuffer(bytes(b), dtype=np.uint8)
+    return out.reshape(p.shape[:-1] + (48,))
+
+
+def g1_decompress(b):
+    """Inverse of g1_compress for one 48-byte string: affine (x, y) integers, or None for
+    infinity.  y = (x^3 + 4)^((p+1)/4) (p = 3 mod 4), sign chosen by the 0x20 flag."""
+    b = bytes(bytearray(b))
+    assert b[0] & 0x80
+    if b[0] & 0x40:
+        return None
+    x = int.from_bytes(bytes([b[0] & 0x1F]) + b[1:], "big")
+    y = pow((x ** 3 + 4) % P_MOD, (P_MOD + 1) // 4, P_MOD)
+    assert y * y % P_MOD == (x ** 3 + 4) % P_MOD
+    if (y > P_MOD - y) != bool(b[0] & 0x20):
+        y = P_MOD - y
+    return x, y

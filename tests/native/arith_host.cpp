@@ -108,6 +108,15 @@ void host_g1_add_aff(uint64_t *r, const uint64_t *p, const uint64_t *q) {
     stp(r, a);
 }
 
+// 48-byte compressed serialisation of a g1p_t.
+void host_g1_compress(uint8_t *out, const uint64_t *p) {
+    g1p a;
+    ldp(a, p);
+    uint32_t w[12];
+    g1_compress(w, a);
+    memcpy(out, w, 48);
+}
+
 void host_g1_dbl(uint64_t *r, const uint64_t *p) {
     g1p a;
     ldp(a, p);

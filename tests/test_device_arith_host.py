@@ -145,3 +145,15 @@ def test_g1_chain_lazy(L):
         a = 2 * a + b
         b = a - b
     assert O.g1p_eq(r, O.g1p_op("mul", G, O.fr(b % R)))
+
+
+def test_g1_compress_against_oracle(L):
+    """The device serialisation (g1_compress) equals the oracle's for G, -G, infinity and
+    random multiples, from non-normalised projective inputs."""
+    G = O.g1p_gen()
+    pts = [G, O.g1p_op("neg", G), O.g1p_inf()] + [O.g1p_op("mul", G, k) for k in S.random_fr(20, 31)]
+    for p in pts:
+        p = np.ascontiguousarray(p, dtype=np.uint64)
+        out = np.zeros(48, dtype=np.uint8)
+        L.host_g1_compress(out.ctypes.data_as(ctypes.c_void_p), p64(p))
+        assert bytes(out) == bytes(O.g1_compress(p))

@@ -14,7 +14,7 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 def header_functions():
     text = open(os.path.join(ROOT, "include", "fk20.h")).read()
     text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
-    return sorted(set(re.findall(r"^\s*(?:int|const char \*)\s*\**\s*(\w+)\s*\(", text, flags=re.M)))
+    return sorted(set(re.findall(r"^\s*(?:int|size_t|const char \*)\s*\**\s*(\w+)\s*\(", text, flags=re.M)))
 
 
 def test_header_parsed():
@@ -64,22 +64,33 @@ def test_zero_rows_is_a_no_op():
 def test_twiddle_programs_selfcheck():
     """GLV split + wNAF programs of every w^(+-i), i < 512, evaluate to the root over Fr."""
     from fk20_amd import prover
-    status, nops = prover.twiddle_selfcheck()
-    assert status == 0
-    # about 2 x 26 additions per program (wNAF-4 digits of two 128-bit halves)
-    assert 1024 * 40 < nops < 1024 * 70
+    status, nops, width = prover.twiddle_selfcheck()
+    assert status == 0 and width in (3, 4, 5)
+    # about 2 x 128 / (width + 1) additions per program (wNAF digits of two 128-bit halves)
+    assert 1024 * 0.8 * 256 / (width + 1) < nops < 1024 * 1.3 * 256 / (width + 1)
 
 
 def test_setup_table_bytes():
     from fk20_amd import prover
     assert prover.setup_table_bytes(4096, 8) == 16 * 512 * 32 * 128 * 96
     assert prover.setup_table_bytes(1024, 10) == 16 * 128 * 26 * 512 * 96
-    assert prover.setup_table_bytes(4096, 9) == 0
+    assert prover.setup_table_bytes(4096, 13) == 16 * 512 * 20 * 4096 * 96
+    assert prover.setup_table_bytes(4096, 7) == 0
+    assert prover.setup_table_bytes(4096, 14) == 0
 
 
 def test_executed_work_model_matches_library_programs():
     """The Python work model rebuilds the same twiddle programs the library runs."""
     from fk20_amd import prover, workload
-    _, nops = prover.twiddle_selfcheck()
+    _, nops, width = prover.twiddle_selfcheck()
+    assert workload.WNAF_W == width
     model = sum(workload.twiddle_program(e, inv)[3] for inv in (False, True) for e in range(512))
     assert model == nops
+
+
+def test_output_format_sizes_and_argument_checks():
+    lib = L.lib()
+    assert [lib.fk20_output_point_bytes(f) for f in (0, 1, 2, 3, 4, -1)] == [144, 144, 48, 48, 0, 0]
+    assert lib.fk20_ctx_set_output(None, 0) == -1  # FK20_EINVAL
+    assert lib.fk20_g1p_compress(None, None, 1, None) == -1
+    assert lib.fk20_g1p_compress(None, None, 0, None) == 0

@@ -292,3 +292,28 @@ def test_planted_sign_flip_is_detected():
     fwd = O.fr_fft(vals)
     assert ints(O.fr_fft(fwd, inverse=True)) == ints(vals)
     assert ints(O.fr_fft(O.fr_fft(fwd))) != ints(vals)
+
+
+def test_compressed_serialisation_pins():
+    """Compressed G1: the standard encodings of G (0x97f1d3a7...) and -G (0xb7f1...), infinity
+    (0xc0 00..), and decompression round trips on random points, which fixes the sign flag."""
+    G = O.g1p_gen()
+    g = bytes(O.g1_compress(G))
+    assert g.hex() == ("97f1d3a73197d7942695638c4fa9ac0fc3688c4f9774b905a14e3a3f171bac58"
+                       "6c55e83ff97a1aeffb3af00adb22c6bb")
+    assert bytes(O.g1_compress(O.g1p_op("neg", G)))[0] == 0xB7
+    assert bytes(O.g1_compress(O.g1p_inf())) == b"\xc0" + b"\x00" * 47
+    ks = S.random_fr(12, 77)
+    pts = np.array([O.g1p_op("mul", G, k) for k in ks])
+    comp = O.g1_compress(pts)
+    aff = O.to_affine(pts)
+    for c, a in zip(comp, aff):
+        x, y = O.g1_decompress(c)
+        assert (x, y) == (O.limbs_to_int(a[:6]), O.limbs_to_int(a[6:]))
+
+
+def test_reverse_bit_order_matches_definition():
+    x = np.arange(16 * 3).reshape(1, 16, 3)
+    r = O.reverse_bit_order(x)
+    assert r[0, 1, 0] == x[0, 8, 0] and r[0, 3, 0] == x[0, 12, 0]
+    assert np.array_equal(O.reverse_bit_order(r), x)  # idempotent (kzg_proofs.py:114)
