@@ -354,6 +354,63 @@ def bench_micro(args):
                       "kernels": out}))
 
 
+# ---------------------------------------------------------------- commitments (the step before the path)
+
+def bench_commit(args):
+    """KZG commitments of `rows` polynomials of N coefficients (FK20Py commit_to_poly) over
+    window tables of the SRS, timed like the batched path; a bounded sample checked against
+    the closed form [f(tau)]G on the host."""
+    rows, N = args.rows, args.N
+    log("commit: %d rows, N=%d, window %d" % (rows, N, args.window))
+    tau = S.random_secret(args.seed + 1)
+    pw, acc = [], 1
+    for _ in range(N):
+        pw.append(S.int_to_fr(acc))
+        acc = acc * tau % R_MOD
+    gens = L.DeviceBuffer.from_array(np.tile(G1_GEN, (N, 1)))
+    ks = L.DeviceBuffer.from_array(np.array(pw))
+    srs = L.DeviceBuffer(N * 144)
+    L.call("fk20_g1p_op", 3, srs.ptr, gens.ptr, None, ks.ptr, N, None)
+    L.synchronize()
+    prover = Prover()
+    t0 = time.perf_counter()
+    sid = prover.load_commit_setup(srs, N, args.window)
+    setup_s = time.perf_counter() - t0
+    for b in (gens, ks, srs):
+        b.free()
+    poly_host = S.random_polys(rows, N, args.seed)
+    poly = L.DeviceBuffer.from_array(poly_host)
+    out = L.DeviceBuffer(rows * 144)
+    stream = L.Stream()
+    ms = time_calls(lambda: prover.commit_device(sid, out, poly, rows, stream=stream.handle), args.steps, stream)
+    got = out.download((rows, 18))
+    exact = None
+    if not args.no_cpu:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import oracle as O  # the checker
+        exact = all(np.array_equal(O.to_affine(got[r]), O.to_affine(O.commit_closed_form(poly_host[r], tau)))
+                    for r in range(min(rows, 8)))
+    p50 = statistics.median(ms)
+    exe = N * ((256 + args.window - 1) // args.window) * WL.MIXED_ADD * rows
+    value = rows / (p50 / 1e3)
+    print(json.dumps({
+        "metric": "KZG commitments per second (N=%d, %d rows)" % (N, rows), "value": round(value, 2),
+        "unit": "rows/s", "n_gpus": 1, "steps": args.steps, "ms_per_step": round(p50, 3),
+        "higher_is_better": True, "dtype": "u32",
+        "data": "synthetic: uniform Fr coefficients (PCG64 seed %d), SRS of a seeded secret" % args.seed,
+        "config": {"workload": "kzg_commit", "rows": rows, "N": N, "window_bits": args.window},
+        "setup_table_build_s": round(setup_s, 3),
+        "roofline": {"bound": "valu", "achieved": round(exe * WL.FP_MUL32 / (p50 / 1e3) / 1e12, 4),
+                     "peak": round(WL.PEAK_MUL32_PER_S / 1e12, 2), "unit": "Tmul32/s",
+                     "frac": round(exe * WL.FP_MUL32 / (p50 / 1e3) / WL.PEAK_MUL32_PER_S, 4),
+                     "work": "executed mixed additions (11 Fp mul each) x 300 mul32; reductions excluded"},
+        "bit_exact_rows_checked": 0 if exact is None else min(rows, 8), "bit_exact": exact}))
+    prover.close()
+    poly.free()
+    out.free()
+    stream.destroy()
+
+
 def bench_serve(args):
     """Poisson arrivals of one-row blobs, N = 1024/2048/4096 weighted 1:2:5, offered load swept."""
     import threading
@@ -485,7 +542,7 @@ def main():
                     help="transform stages as one persistent dependency-driven launch (default: one launch per stage)")
     ap.add_argument("--uneven-rows", type=int, default=512,
                     help="rows per size in the uneven N=1024/2048/4096 batch (0: skip)")
-    ap.add_argument("--workload", default="batch", choices=["batch", "serve", "micro"])
+    ap.add_argument("--workload", default="batch", choices=["batch", "serve", "micro", "commit"])
     ap.add_argument("--calibrate-copy-mb", type=int, default=0,
                     help="before timing, one device-to-device copy of this many MiB (the known-size "
                          "kernel that scales FETCH_SIZE / WRITE_SIZE counter runs)")
@@ -512,6 +569,9 @@ def main():
         return
     if args.workload == "micro":
         bench_micro(args)
+        return
+    if args.workload == "commit":
+        bench_commit(args)
         return
 
     if args.calibrate_copy_mb:

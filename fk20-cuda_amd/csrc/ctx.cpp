@@ -139,6 +139,7 @@ hipError_t build_programs(Programs &P) {
 
 struct Setup {
     bool live = false;
+    bool commit = false;  // an SRS for KZG commitments (fk20_commit_setup_load)
     unsigned N = 0, n2 = 0, win = 0, nwin = 0;
     uint32_t *table = nullptr;
     uint32_t *inf = nullptr;
@@ -271,45 +272,102 @@ int fk20_ctx_set_graph(fk20_ctx_t *c, int enable) {
     return FK20_OK;
 }
 
-int fk20_setup_load(fk20_ctx_t *c, const fk20_g1p_t *xext_fft, unsigned N, unsigned window, int *setup_id,
-                    fk20_stream_t stream) {
-    if (!c || !xext_fft || !setup_id || !valid_N(N) || !valid_win(window)) return FK20_EINVAL;
-    std::lock_guard<std::mutex> lk(c->mu);
-    hipStream_t s = reinterpret_cast<hipStream_t>(stream);
-    Setup st;
-    st.N = N;
-    st.n2 = N / 8;
-    st.win = window;
-    st.nwin = nwin_of(window);
-    const unsigned nbases = 16 * st.n2, H = 1u << (window - 1);
-    const unsigned logn2 = st.n2 == 128 ? 7 : st.n2 == 256 ? 8 : 9;
+static hipError_t grow_work(fk20_ctx *c, size_t bytes);
+
+// Window tables of nbases points (standard-form g1p_t, device), each first multiplied by the
+// standard-form scalar `scale` (device, 8 words).  Fills st.table / st.inf.
+static hipError_t build_tables(Setup &st, const fk20_g1p_t *points, unsigned nbases, const uint32_t *scale,
+                               hipStream_t s) {
+    const unsigned H = 1u << (st.win - 1);
     uint32_t *xs = nullptr, *zt = nullptr;
     // bases per build launch, bounding the batch-inversion scratch to ~2 GiB (one thread per
     // (base, window) builds its 2^(w-1) entries, so larger chunks mean more parallel threads)
     const size_t per_base = (size_t)st.nwin * H * 96;
     unsigned chunk = (unsigned)std::max<size_t>(1, ((size_t)2 << 30) / per_base);
     if (chunk > nbases) chunk = nbases;
-    hipError_t e = hipMalloc(&st.table, fk20_setup_table_bytes(N, window));
+    hipError_t e = hipMalloc(&st.table, (size_t)nbases * per_base);
     if (e == hipSuccess) e = hipMalloc(&st.inf, nbases * 4);
     if (e == hipSuccess) e = hipMalloc(&xs, (size_t)nbases * 144);
     if (e == hipSuccess) e = hipMalloc(&zt, (size_t)chunk * per_base);
     if (e == hipSuccess)
-        e = launch_scale_bases(xs, st.inf, reinterpret_cast<const uint32_t *>(xext_fft), nbases,
-                               c->T->inv2k_std[logn2], s);
+        e = launch_scale_bases(xs, st.inf, reinterpret_cast<const uint32_t *>(points), nbases, scale, s);
     for (unsigned b0 = 0; e == hipSuccess && b0 < nbases; b0 += chunk)
-        e = launch_table_build(st.table, zt, xs, st.inf, b0, std::min(chunk, nbases - b0), window, st.nwin, s);
+        e = launch_table_build(st.table, zt, xs, st.inf, b0, std::min(chunk, nbases - b0), st.win, st.nwin, s);
     if (e == hipSuccess) e = hipStreamSynchronize(s);
     if (xs) (void)hipFree(xs);
     if (zt) (void)hipFree(zt);
     if (e != hipSuccess) {
         if (st.table) (void)hipFree(st.table);
         if (st.inf) (void)hipFree(st.inf);
-        return status_of(e);
+        st.table = st.inf = nullptr;
     }
+    return e;
+}
+
+int fk20_setup_load(fk20_ctx_t *c, const fk20_g1p_t *xext_fft, unsigned N, unsigned window, int *setup_id,
+                    fk20_stream_t stream) {
+    if (!c || !xext_fft || !setup_id || !valid_N(N) || !valid_win(window)) return FK20_EINVAL;
+    std::lock_guard<std::mutex> lk(c->mu);
+    Setup st;
+    st.N = N;
+    st.n2 = N / 8;
+    st.win = window;
+    st.nwin = nwin_of(window);
+    const unsigned logn2 = st.n2 == 128 ? 7 : st.n2 == 256 ? 8 : 9;
+    // the bases carry the IFFT's 1/n2
+    hipError_t e = build_tables(st, xext_fft, 16 * st.n2, c->T->inv2k_std[logn2], reinterpret_cast<hipStream_t>(stream));
+    if (e != hipSuccess) return status_of(e);
     st.live = true;
     c->setups.push_back(st);
     *setup_id = (int)c->setups.size() - 1;
     return FK20_OK;
+}
+
+int fk20_commit_setup_load(fk20_ctx_t *c, const fk20_g1p_t *srs, unsigned N, unsigned window, int *setup_id,
+                           fk20_stream_t stream) {
+    if (!c || !srs || !setup_id || !valid_N(N) || !valid_win(window)) return FK20_EINVAL;
+    std::lock_guard<std::mutex> lk(c->mu);
+    Setup st;
+    st.commit = true;
+    st.N = N;
+    st.n2 = N / 8;
+    st.win = window;
+    st.nwin = nwin_of(window);
+    // tw_std[0] = w^0 = 1 (standard form): the SRS points are taken as they are
+    hipError_t e = build_tables(st, srs, N, c->T->tw_std[0], reinterpret_cast<hipStream_t>(stream));
+    if (e != hipSuccess) return status_of(e);
+    st.live = true;
+    c->setups.push_back(st);
+    *setup_id = (int)c->setups.size() - 1;
+    return FK20_OK;
+}
+
+size_t fk20_commit_setup_table_bytes(unsigned N, unsigned window) {
+    if (!valid_N(N) || !valid_win(window)) return 0;
+    return (size_t)N * nwin_of(window) * ((size_t)1 << (window - 1)) * 96;
+}
+
+size_t fk20_commit_workspace_bytes(unsigned rows, unsigned N) {
+    if (!valid_N(N)) return 0;
+    return commit_workspace_bytes(rows, N);
+}
+
+int fk20_commit_rows(fk20_ctx_t *c, int setup_id, fk20_g1p_t *commitments, const fk20_fr_t *polynomial,
+                     unsigned rows, fk20_stream_t stream) {
+    if (!c) return FK20_EINVAL;
+    std::lock_guard<std::mutex> lk(c->mu);
+    if (setup_id < 0 || setup_id >= (int)c->setups.size() || !c->setups[setup_id].live ||
+        !c->setups[setup_id].commit)
+        return FK20_EINVAL;
+    if (!rows) return FK20_OK;
+    if (!commitments || !polynomial) return FK20_EINVAL;
+    const Setup &st = c->setups[setup_id];
+    hipError_t e = grow_work(c, commit_workspace_bytes(rows, st.N));
+    if (e != hipSuccess) return status_of(e);
+    return status_of(launch_commit(reinterpret_cast<uint32_t *>(commitments),
+                                   reinterpret_cast<const uint32_t *>(polynomial), st.table, st.inf,
+                                   reinterpret_cast<uint32_t *>(c->work), rows, st.N, st.win,
+                                   reinterpret_cast<hipStream_t>(stream)));
 }
 
 int fk20_setup_unload(fk20_ctx_t *c, int id) {
@@ -408,7 +466,8 @@ int fk20_prove_batch(fk20_ctx_t *c, const fk20_batch_item_t *items, unsigned nit
     std::vector<fk20_batch_item_t> v;
     for (unsigned k = 0; k < nitems; k++) {
         const fk20_batch_item_t &it = items[k];
-        if (it.setup_id < 0 || it.setup_id >= (int)c->setups.size() || !c->setups[it.setup_id].live)
+        if (it.setup_id < 0 || it.setup_id >= (int)c->setups.size() || !c->setups[it.setup_id].live ||
+            c->setups[it.setup_id].commit)
             return FK20_EINVAL;
         if (!it.rows) continue;
         if (!it.polynomial || !it.h_fft) return FK20_EINVAL;
@@ -456,7 +515,8 @@ int fk20_prove_batch_ws(fk20_ctx_t *c, const fk20_batch_item_t *items, unsigned 
     std::vector<fk20_batch_item_t> v;
     for (unsigned k = 0; k < nitems; k++) {
         const fk20_batch_item_t &it = items[k];
-        if (it.setup_id < 0 || it.setup_id >= (int)c->setups.size() || !c->setups[it.setup_id].live)
+        if (it.setup_id < 0 || it.setup_id >= (int)c->setups.size() || !c->setups[it.setup_id].live ||
+            c->setups[it.setup_id].commit)
             return FK20_EINVAL;
         if (!it.rows) continue;
         if (!it.polynomial || !it.h_fft) return FK20_EINVAL;

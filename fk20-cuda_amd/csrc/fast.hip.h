@@ -32,6 +32,10 @@ hipError_t launch_scale_bases(uint32_t *xs, uint32_t *inf, const uint32_t *xext,
                               const uint32_t *scale, hipStream_t s);
 hipError_t launch_table_build(uint32_t *table, uint32_t *zt, const uint32_t *xs, const uint32_t *inf,
                               unsigned base0, unsigned nb, unsigned win_bits, unsigned nwin, hipStream_t s);
+// KZG commitments of `rows` polynomials of length N over an SRS setup's window tables.
+size_t commit_workspace_bytes(unsigned rows, unsigned N);
+hipError_t launch_commit(uint32_t *out, const uint32_t *poly, const uint32_t *table, const uint32_t *inf,
+                         uint32_t *work, unsigned rows, unsigned N, unsigned win_bits, hipStream_t s);
 // One group of rows sharing a setup (hence N) and buffers; the unit of a batched launch.
 static constexpr unsigned MAX_ITEMS = 16;
 struct Item {

@@ -234,13 +234,40 @@ __global__ void __launch_bounds__(256) k_toeplitz_fft_multi(Launch L, const DevT
 
 // ---------------------------------------------------------------- window-table MSM
 
+// acc += k * B for a 256-bit standard-form scalar k (8 words, consumed) by signed fixed
+// windows of WIN bits over B's table tb[win][d - 1] = d * 2^(WIN*win) * B (affine): one
+// mixed addition per non-zero digit, no doublings.
+__device__ __forceinline__ void window_term(g1p &acc, uint32_t *k, const uint32_t *tb, unsigned WIN) {
+    const unsigned NWIN = (256 + WIN - 1) / WIN, H = 1u << (WIN - 1), MASK = (1u << WIN) - 1;
+    uint32_t carry = 0;
+#pragma unroll 1
+    for (unsigned win = 0; win < NWIN; win++) {
+        const uint32_t bits = (k[0] & MASK) + carry;
+        carry = bits > H ? 1u : 0u;
+        const int d = (int)bits - (int)(carry << WIN);
+        // shift the scalar right by WIN bits (WIN < 32)
+#pragma unroll
+        for (int q = 0; q < 7; q++) k[q] = (k[q] >> WIN) | (k[q + 1] << (32 - WIN));
+        k[7] >>= WIN;
+        if (d != 0) {
+            const unsigned e = (unsigned)(d < 0 ? -d : d) - 1;
+            const uint32_t *ent = tb + ((size_t)win * H + e) * G1A_WORDS;
+            fe x, y;
+            fe_load(x, ent);
+            fe_load(y, ent + 12);
+            if (d < 0) fe_neg<2>(y, y);
+            g1_add_aff(acc, acc, x, y);
+        }
+    }
+}
+
 // hext_fft[j] for 64 rows of one column j: sum over 16 terms of ceil(256/w) signed-window
 // digits, one mixed addition each.  Output to D[bitrev(j)][row].
 __global__ void __launch_bounds__(64) k_msm_multi(Launch L) {
     unsigned local;
     const Item &it = L.item[find_item(L, blockIdx.x, local)];
     const unsigned rows = it.rows, n2 = 1u << it.logn2, rg = (rows + 63) / 64;
-    const unsigned WIN = it.win, NWIN = (256 + WIN - 1) / WIN, H = 1u << (WIN - 1), MASK = (1u << WIN) - 1;
+    const unsigned WIN = it.win, NWIN = (256 + WIN - 1) / WIN, H = 1u << (WIN - 1);
     const unsigned j = local / rg;
     const unsigned row = (local % rg) * 64 + threadIdx.x;
     if (row >= rows) return;
@@ -252,29 +279,61 @@ __global__ void __launch_bounds__(64) k_msm_multi(Launch L) {
         if (it.inf[b]) continue;
         uint32_t k[8];
         ld_words(k, it.tcT + (((size_t)j * 16 + i) * rows + row) * FR_WORDS, 2);
-        const uint32_t *tb = it.table + (size_t)b * NWIN * H * G1A_WORDS;
-        uint32_t carry = 0;
-#pragma unroll 1
-        for (unsigned win = 0; win < NWIN; win++) {
-            const uint32_t bits = (k[0] & MASK) + carry;
-            carry = bits > H ? 1u : 0u;
-            const int d = (int)bits - (int)(carry << WIN);
-            // shift the scalar right by WIN bits (WIN < 32)
-#pragma unroll
-            for (int q = 0; q < 7; q++) k[q] = (k[q] >> WIN) | (k[q + 1] << (32 - WIN));
-            k[7] >>= WIN;
-            if (d != 0) {
-                const unsigned e = (unsigned)(d < 0 ? -d : d) - 1;
-                const uint32_t *ent = tb + ((size_t)win * H + e) * G1A_WORDS;
-                fe x, y;
-                fe_load(x, ent);
-                fe_load(y, ent + 12);
-                if (d < 0) fe_neg<2>(y, y);
-                g1_add_aff(acc, acc, x, y);
-            }
-        }
+        window_term(acc, k, it.table + (size_t)b * NWIN * H * G1A_WORDS, WIN);
     }
     pt_store(it.D + ((size_t)bitrev(j, it.logn2) * rows + row) * 36, acc);
+}
+
+// ---------------------------------------------------------------- KZG commitments
+//
+// commit[row] = sum_i poly[row][i] * srs[i] (FK20Py kzg_proofs.py:159 commit_to_poly) over
+// window tables of the SRS points built like the FK20 setup's.  Lanes are rows; a block sums
+// one chunk of consecutive coefficients into partial[chunk][row], then two reduction passes
+// add the chunks.
+
+__global__ void __launch_bounds__(64) k_commit_partial(const uint32_t *poly, const uint32_t *table,
+                                                       const uint32_t *inf, uint32_t *partial, unsigned rows,
+                                                       unsigned N, unsigned win_bits, unsigned chunk) {
+    const unsigned rg = (rows + 63) / 64, c = blockIdx.x / rg;
+    const unsigned row = (blockIdx.x % rg) * 64 + threadIdx.x;
+    if (row >= rows) return;
+    const unsigned NWIN = (256 + win_bits - 1) / win_bits, H = 1u << (win_bits - 1);
+    const unsigned i0 = c * chunk, i1 = min(i0 + chunk, N);
+    g1p acc;
+    g1_inf(acc);
+#pragma unroll 1
+    for (unsigned i = i0; i < i1; i++) {
+        if (inf[i]) continue;
+        uint32_t k[8];
+        ld_words(k, poly + ((size_t)row * N + i) * FR_WORDS, 2);
+        window_term(acc, k, table + (size_t)i * NWIN * H * G1A_WORDS, win_bits);
+    }
+    pt_store(partial + ((size_t)c * rows + row) * 36, acc);
+}
+
+// out[q][row] = sum of in[q*group .. min((q+1)*group, nparts)][row]; with std_out the single
+// group's sum goes to out[row] as a canonical standard-form g1p_t.
+__global__ void __launch_bounds__(64) k_commit_reduce(const uint32_t *in, uint32_t *out, unsigned rows,
+                                                      unsigned nparts, unsigned group, unsigned nout,
+                                                      unsigned std_out) {
+    const size_t t = (size_t)blockIdx.x * 64 + threadIdx.x;
+    if (t >= (size_t)rows * nout) return;
+    const unsigned row = (unsigned)(t % rows), q = (unsigned)(t / rows);
+    const unsigned p0 = q * group, p1 = min(p0 + group, nparts);
+    g1p acc, x;
+    g1_inf(acc);
+#pragma unroll 1
+    for (unsigned p = p0; p < p1; p++) {
+        pt_load(x, in + ((size_t)p * rows + row) * 36);
+        g1_add(acc, acc, x);
+    }
+    if (std_out) {
+        uint32_t w[36];
+        g1_store_std(w, acc);
+        st_words(out + (size_t)row * G1P_WORDS, w, 9);
+    } else {
+        pt_store(out + ((size_t)q * rows + row) * 36, acc);
+    }
 }
 
 // ---------------------------------------------------------------- twiddle programs
@@ -550,6 +609,31 @@ hipError_t launch_table_build(uint32_t *table, uint32_t *zt, const uint32_t *xs,
     const unsigned threads = nb * nwin;
     hipLaunchKernelGGL(k_table_build, dim3((threads + 63) / 64), dim3(64), 0, s, table, zt, xs, inf, base0, nb,
                        win_bits, nwin);
+    return hipGetLastError();
+}
+
+// Commitment chunking: coefficients per partial sum, partials per first reduction group.
+static constexpr unsigned COMMIT_CHUNK = 16, COMMIT_GROUP = 16;
+
+size_t commit_workspace_bytes(unsigned rows, unsigned N) {
+    const size_t nparts = (N + COMMIT_CHUNK - 1) / COMMIT_CHUNK;
+    const size_t nmid = (nparts + COMMIT_GROUP - 1) / COMMIT_GROUP;
+    return (nparts + nmid) * rows * 144;
+}
+
+hipError_t launch_commit(uint32_t *out, const uint32_t *poly, const uint32_t *table, const uint32_t *inf,
+                         uint32_t *work, unsigned rows, unsigned N, unsigned win_bits, hipStream_t s) {
+    if (!rows) return hipSuccess;
+    const unsigned nparts = (N + COMMIT_CHUNK - 1) / COMMIT_CHUNK;
+    const unsigned nmid = (nparts + COMMIT_GROUP - 1) / COMMIT_GROUP;
+    const unsigned rg = (rows + 63) / 64;
+    uint32_t *partial = work, *mid = work + (size_t)nparts * rows * 36;
+    hipLaunchKernelGGL(k_commit_partial, dim3(nparts * rg), dim3(64), 0, s, poly, table, inf, partial, rows, N,
+                       win_bits, COMMIT_CHUNK);
+    const size_t t1 = (size_t)rows * nmid;
+    hipLaunchKernelGGL(k_commit_reduce, dim3((unsigned)((t1 + 63) / 64)), dim3(64), 0, s, partial, mid, rows, nparts,
+                       COMMIT_GROUP, nmid, 0u);
+    hipLaunchKernelGGL(k_commit_reduce, dim3((rows + 63) / 64), dim3(64), 0, s, mid, out, rows, nmid, nmid, 1u, 1u);
     return hipGetLastError();
 }
 

@@ -56,6 +56,10 @@ _SIGS = {
     "fk20_ctx_get_output": ([_P, ctypes.POINTER(_I)], _I),
     "fk20_output_point_bytes": ([_I], _SZ),
     "fk20_g1p_compress": ([_P, _P, _SZ, _P], _I),
+    "fk20_commit_setup_load": ([_P, _P, _U, _U, ctypes.POINTER(_I), _P], _I),
+    "fk20_commit_setup_table_bytes": ([_U, _U], _SZ),
+    "fk20_commit_workspace_bytes": ([_U, _U], _SZ),
+    "fk20_commit_rows": ([_P, _I, _P, _P, _U, _P], _I),
     "fk20_setup_table_bytes": ([_U, _U], _SZ),
     "fk20_setup_load": ([_P, _P, _U, _U, ctypes.POINTER(_I), _P], _I),
     "fk20_setup_unload": ([_P, _I], _I),

@@ -38,6 +38,37 @@ class Prover:
         self.setups[sid.value] = N
         return sid.value
 
+    def load_commit_setup(self, srs, N, window=8, stream=None):
+        """KZG commitment setup from the first N SRS points: host (>= N, 18) or a DeviceBuffer."""
+        buf = srs if isinstance(srs, DeviceBuffer) else DeviceBuffer.from_array(
+            np.ascontiguousarray(srs[:N], dtype=np.uint64))
+        sid = ctypes.c_int()
+        call("fk20_commit_setup_load", self.ctx, buf.ptr, N, window, ctypes.byref(sid), stream)
+        if buf is not srs:
+            buf.free()
+        self.setups[sid.value] = N
+        return sid.value
+
+    def commit_device(self, sid, out, poly, rows, stream=None):
+        """Enqueue commitments of rows: poly (rows, N) fr_t and out (rows) g1p_t device buffers."""
+        op = out.ptr if isinstance(out, DeviceBuffer) else out
+        pp = poly.ptr if isinstance(poly, DeviceBuffer) else poly
+        call("fk20_commit_rows", self.ctx, sid, op, pp, rows, stream)
+
+    def commit(self, sid, poly):
+        """Host polynomials (rows, N, 4) -> host commitments (rows, 18)."""
+        poly = np.ascontiguousarray(poly, dtype=np.uint64)
+        rows, N = poly.shape[0], poly.shape[1]
+        assert self.setups[sid] == N
+        pb = DeviceBuffer.from_array(poly)
+        out = DeviceBuffer(max(1, rows) * 144)
+        self.commit_device(sid, out, pb, rows)
+        synchronize()
+        res = out.download((rows, 18))
+        pb.free()
+        out.free()
+        return res
+
     def unload_setup(self, sid):
         call("fk20_setup_unload", self.ctx, sid)
         self.setups.pop(sid, None)

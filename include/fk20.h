@@ -174,6 +174,19 @@ int fk20_prove_batch(fk20_ctx_t *ctx, const fk20_batch_item_t *items, unsigned n
  * fk20_rows_workspace_bytes over the items (no allocation; used by the serving loop). */
 int fk20_prove_batch_ws(fk20_ctx_t *ctx, const fk20_batch_item_t *items, unsigned nitems,
                         void *workspace, size_t workspace_bytes, fk20_stream_t stream);
+/* KZG commitments of rows (the step before the path): commitments[r] = sum_{i<N}
+ * polynomial[r][i] * srs[i], as FK20Py commit_to_poly (kzg_proofs.py:159, lincomb over
+ * setup[0][:len(polynomial)]).  fk20_commit_setup_load builds fixed-base window tables of the
+ * N SRS points srs[0..N) (device, standard-form g1p_t; synchronises `stream`); the returned
+ * id is accepted only by fk20_commit_rows, which writes one canonical standard-form g1p_t per
+ * row using the context workspace.  No CUDA counterpart: the reference's commitments are
+ * FK20Py-only. */
+int fk20_commit_setup_load(fk20_ctx_t *ctx, const fk20_g1p_t *srs, unsigned N, unsigned window,
+                           int *setup_id, fk20_stream_t stream);
+size_t fk20_commit_setup_table_bytes(unsigned N, unsigned window);
+size_t fk20_commit_workspace_bytes(unsigned rows, unsigned N);
+int fk20_commit_rows(fk20_ctx_t *ctx, int setup_id, fk20_g1p_t *commitments, const fk20_fr_t *polynomial,
+                     unsigned rows, fk20_stream_t stream);
 /* N and window of a loaded setup. */
 int fk20_setup_info(fk20_ctx_t *ctx, int setup_id, unsigned *N, unsigned *window);
 

@@ -800,6 +800,23 @@ void or_setup_srs(u64 *srs, const u64 tau[4], int count) {
     }
 }
 
+/* KZG commitment of each row: commit[r] = sum_i poly[r][i] * srs[i] (FK20Py kzg_proofs.py:159
+ * commit_to_poly = lincomb(setup[0][:len(polynomial)], polynomial)), by double-and-add. */
+void or_commit(u64 *commit, const u64 *poly, const u64 *srs, int N, int rows) {
+    ensure_init();
+    for (int r = 0; r < rows; r++) {
+        gp acc;
+        gp_inf(&acc);
+        for (int i = 0; i < N; i++) {
+            gp base, t;
+            gp_from_std(&base, srs + 18 * (size_t)i);
+            gp_mul(&t, &base, poly + 4 * ((size_t)r * N + i));
+            gp_add(&acc, &acc, &t);
+        }
+        gp_to_std(commit + 18 * (size_t)r, &acc);
+    }
+}
+
 /* fk20_multi.py:82-85 + fk20_single.py:45-58 (toeplitz_part1) */
 void or_setup2xext_fft(u64 *xext_fft, const u64 *srs, int N) {
     ensure_init();

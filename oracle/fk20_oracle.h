@@ -82,6 +82,8 @@ void or_fk20_poly2h_fft(uint64_t *h_fft, const uint64_t *poly, const uint64_t *x
 /* ---- Setup ---- */
 /* srs[i] = [tau^i] G for i < count, normalised (z = 1). */
 void or_setup_srs(uint64_t *srs, const uint64_t tau[4], int count);
+/* commit[rows]: sum_i poly[r][i] * srs[i] (kzg_proofs.py:159 commit_to_poly), i < N. */
+void or_commit(uint64_t *commit, const uint64_t *poly, const uint64_t *srs, int N, int rows);
 /* xext_fft[16][n2] from the SRS by FK20Py toeplitz_part1 (setup[N-l-1-i::-l] + inf, padded). */
 void or_setup2xext_fft(uint64_t *xext_fft, const uint64_t *srs, int N);
 /* Same xext_fft computed as [xhat]G from the secret (scalar shadow); normalised points. */

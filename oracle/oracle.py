@@ -208,6 +208,25 @@ def setup_srs(tau, count):
     return out
 
 
+def commit(poly, srs):
+    """KZG commitments sum_i poly[r][i] * srs[i] of rows (rows, N, 4) over srs (>= N, 18):
+    FK20Py commit_to_poly (kzg_proofs.py:159), by direct double-and-add."""
+    poly = np.ascontiguousarray(poly, dtype=np.uint64)
+    rows, N = poly.shape[0], poly.shape[1]
+    srs = np.ascontiguousarray(srs, dtype=np.uint64)
+    out = np.zeros((rows, 18), dtype=np.uint64)
+    lib().or_commit(_p(out), _p(poly), _p(srs), ctypes.c_int(N), ctypes.c_int(rows))
+    return out
+
+
+def commit_closed_form(poly_row, tau):
+    """[f(tau)]G for one row: the commitment of a setup with secret tau."""
+    f = 0
+    for c in reversed([limbs_to_int(x) for x in np.asarray(poly_row)]):
+        f = (f * tau + c) % R_MOD
+    return g1p_op("mul", g1p_gen(), fr(f))
+
+
 def setup2xext_fft(srs, N):
     out = np.zeros((16, N // 8, 18), dtype=np.uint64)
     lib().or_setup2xext_fft(_p(out), _p(np.ascontiguousarray(srs, dtype=np.uint64)), ctypes.c_int(N))

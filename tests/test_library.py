@@ -94,3 +94,12 @@ def test_output_format_sizes_and_argument_checks():
     assert lib.fk20_ctx_set_output(None, 0) == -1  # FK20_EINVAL
     assert lib.fk20_g1p_compress(None, None, 1, None) == -1
     assert lib.fk20_g1p_compress(None, None, 0, None) == 0
+
+
+def test_commit_sizes_and_argument_checks():
+    lib = L.lib()
+    assert lib.fk20_commit_setup_table_bytes(4096, 8) == 4096 * 32 * 128 * 96
+    assert lib.fk20_commit_setup_table_bytes(4000, 8) == 0
+    assert lib.fk20_commit_workspace_bytes(512, 4096) == (256 + 16) * 512 * 144
+    assert lib.fk20_commit_setup_load(None, None, 4096, 8, None, None) == -1
+    assert lib.fk20_commit_rows(None, 0, None, None, 1, None) == -1

@@ -317,3 +317,20 @@ def test_reverse_bit_order_matches_definition():
     r = O.reverse_bit_order(x)
     assert r[0, 1, 0] == x[0, 8, 0] and r[0, 3, 0] == x[0, 12, 0]
     assert np.array_equal(O.reverse_bit_order(r), x)  # idempotent (kzg_proofs.py:114)
+
+
+def test_commit_direct_equals_closed_form():
+    """The oracle's commitment (direct linear combination over the SRS, kzg_proofs.py:159)
+    equals [f(tau)]G, the closed form of an SRS with secret tau; a zero row commits to
+    infinity, and a single coefficient c_i gives c_i [tau^i]G."""
+    tau = S.random_secret(61)
+    N = 64
+    srs = O.setup_srs(tau, N)
+    poly = S.random_polys(3, N, 62)
+    poly[1] = 0
+    poly[2] = 0
+    poly[2, 5] = S.random_fr(1, 63)[0]
+    got = O.commit(poly, srs)
+    assert O.g1p_eq(got[0], O.commit_closed_form(poly[0], tau))
+    assert O.g1p_is_inf(got[1])
+    assert O.g1p_eq(got[2], O.g1p_op("mul", srs[5], poly[2, 5]))
