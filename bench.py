@@ -143,23 +143,30 @@ class Batch:
 # ---------------------------------------------------------------- F: batched path
 
 def time_fused(b, prover, sid, steps, warmup):
+    """Step times of the batched path (HIP events around each whole step, the library's default
+    launch layout), then one pass per step with per-launch events for the kernel breakdown
+    (single-stream sequence: launches are timed one after another), then graph replay."""
     stream = L.Stream()
     e0, e1 = L.Event(), L.Event()
-    prover.set_launch_timing(True)
     for _ in range(warmup):
         prover.prove_device(sid, b.h_fft, b.poly, b.rows, stream=stream.handle)
     stream.synchronize()
-    step_ms, launches = [], {}
+    step_ms = []
     L.synchronize()  # the timed steps start and end on an idle device
     for _ in range(steps):
         e0.record(stream)
         prover.prove_device(sid, b.h_fft, b.poly, b.rows, stream=stream.handle)
         e1.record(stream)
         step_ms.append(e0.elapsed_ms(e1))
+    stream.synchronize()
+    L.synchronize()
+    launches = {}
+    prover.set_launch_timing(True)
+    for _ in range(steps):
+        prover.prove_device(sid, b.h_fft, b.poly, b.rows, stream=stream.handle)
         for kind, ms in prover.launch_times():
             launches.setdefault(LAUNCH_KINDS[kind], []).append(ms)
     stream.synchronize()
-    L.synchronize()
     prover.set_launch_timing(False)
     # the same sequence replayed from a hipGraph
     prover.set_graph(True)
@@ -538,6 +545,8 @@ def main():
     ap.add_argument("--cpu-rows", type=int, default=16, help="rows in the CPU baseline sample")
     ap.add_argument("--cpu-threads", type=int, default=16)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--streams", type=int, default=2,
+                    help="launch sequences per batch: 2 halves on two streams (default) or 1")
     ap.add_argument("--chain", action="store_true",
                     help="transform stages as one persistent dependency-driven launch (default: one launch per stage)")
     ap.add_argument("--uneven-rows", type=int, default=512,
@@ -588,6 +597,7 @@ def main():
     b = Batch(args.rows, args.N, args.seed)
     prover = Prover()
     prover.set_chain(args.chain)
+    prover.set_streams(args.streams)
     t0 = time.perf_counter()
     sid = prover.load_setup(b.xext, args.N, args.window)
     setup_s = time.perf_counter() - t0
@@ -692,7 +702,8 @@ def main():
         "data": "synthetic: uniform Fr coefficients (PCG64 seed %d), setup from a seeded secret" % args.seed,
         "config": {"workload": "fk20_batch", "rows": args.rows, "N": args.N, "l": 16, "fft_size": args.N // 8,
                    "form": "F batched (fk20_prove_rows)", "msm_window_bits": args.window,
-                   "transform": "persistent chain" if args.chain else "per-stage launches"},
+                   "transform": "persistent chain" if args.chain else "per-stage launches",
+                   "streams": args.streams},
         "latency_ms": {"step_p50": round(ms, 3), "step_p99": round(pct(step_ms, 99), 3)},
         "forms": forms,
         "uneven_batch": uneven,

@@ -163,6 +163,12 @@ struct fk20_ctx {
     bool use_graph = false;
     bool use_chain = false;  // transform stages as one persistent launch (k_chain)
     int out_format = FK20_OUT_G1P;
+    // fk20_prove_batch splits its rows into `nsub` halves at 64-row boundaries and runs each
+    // half's launch sequence on its own stream, forked from and joined back to the caller's
+    // stream, so one half's stage launches fill the SIMDs the other half's idle at a boundary
+    unsigned nsub = 2;
+    hipStream_t sub[2] = {nullptr, nullptr};
+    hipEvent_t fork = nullptr, join[2] = {nullptr, nullptr};
     std::vector<fk20_batch_item_t> gkey;  // items of the captured graph
     hipStream_t gstream = nullptr;
     hipGraphExec_t gexec = nullptr;
@@ -192,11 +198,12 @@ size_t fk20_setup_table_bytes(unsigned N, unsigned window) {
 }
 
 // Per item: tcT (16 n2 Fr per row), D (n2 points per row), the chain flags (n2 words per
-// 64-row group) and a 256-byte control block (the chain's work counter; item 0's is used).
+// 64-row group) and 256-byte control blocks (the chain's work counter; item 0's is used):
+// two of them, so that the two halves of a split batch fit in the whole's size.
 size_t fk20_rows_workspace_bytes(unsigned rows, unsigned N) {
     if (!valid_N(N)) return 0;
     const size_t n2 = N / 8;
-    return (size_t)rows * n2 * (16 * 32 + 144) + (size_t)((rows + 63) / 64) * n2 * 4 + 256;
+    return (size_t)rows * n2 * (16 * 32 + 144) + (size_t)((rows + 63) / 64) * n2 * 4 + 512;
 }
 
 int fk20_ctx_create(fk20_ctx_t **out) {
@@ -205,6 +212,11 @@ int fk20_ctx_create(fk20_ctx_t **out) {
     hipError_t e = hipGetDevice(&c->device);
     if (e == hipSuccess) c->T = current_device_tables(&e);
     if (e == hipSuccess) e = build_programs(c->prog);
+    for (int k = 0; k < 2 && e == hipSuccess; k++) {
+        e = hipStreamCreateWithFlags(&c->sub[k], hipStreamNonBlocking);
+        if (e == hipSuccess) e = hipEventCreateWithFlags(&c->join[k], hipEventDisableTiming);
+    }
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&c->fork, hipEventDisableTiming);
     if (e != hipSuccess) {
         fk20_ctx_destroy(c);
         return status_of(e);
@@ -225,6 +237,11 @@ int fk20_ctx_destroy(fk20_ctx_t *c) {
     if (c->prog.index) (void)hipFree(c->prog.index);
     if (c->gexec) (void)hipGraphExecDestroy(c->gexec);
     for (hipEvent_t e : c->events) (void)hipEventDestroy(e);
+    for (int k = 0; k < 2; k++) {
+        if (c->sub[k]) (void)hipStreamDestroy(c->sub[k]);
+        if (c->join[k]) (void)hipEventDestroy(c->join[k]);
+    }
+    if (c->fork) (void)hipEventDestroy(c->fork);
     delete c;
     return FK20_OK;
 }
@@ -243,6 +260,18 @@ int fk20_ctx_set_output(fk20_ctx_t *c, int format) {
         c->gkey.clear();
     }
     c->out_format = format;
+    return FK20_OK;
+}
+
+int fk20_ctx_set_streams(fk20_ctx_t *c, unsigned n) {
+    if (!c || n < 1 || n > 2) return FK20_EINVAL;
+    std::lock_guard<std::mutex> lk(c->mu);
+    if (c->nsub != n && c->gexec) {
+        (void)hipGraphExecDestroy(c->gexec);  // captured with the other stream layout
+        c->gexec = nullptr;
+        c->gkey.clear();
+    }
+    c->nsub = n;
     return FK20_OK;
 }
 
@@ -426,12 +455,13 @@ static bool same_items(const std::vector<fk20_batch_item_t> &a, const fk20_batch
     return true;
 }
 
-static hipError_t enqueue_items(fk20_ctx *c, const fk20_batch_item_t *items, unsigned n, hipStream_t s,
-                                LaunchHook hook, void *workspace = nullptr) {
+// Carves each item's workspace (tcT, D, chain flags, control block) from w and launches the
+// batch on s.
+static hipError_t enqueue_launch(fk20_ctx *c, const fk20_batch_item_t *items, unsigned n, hipStream_t s,
+                                 LaunchHook hook, char *w) {
     Launch L;
     memset(&L, 0, sizeof L);
     L.nitems = n;
-    char *w = reinterpret_cast<char *>(workspace ? workspace : c->work);
     uint32_t *ctl = nullptr;
     for (unsigned k = 0; k < n; k++) {
         const Setup &st = c->setups[items[k].setup_id];
@@ -446,7 +476,7 @@ static hipError_t enqueue_items(fk20_ctx *c, const fk20_batch_item_t *items, uns
         it.flags = reinterpret_cast<uint32_t *>(w);
         w += (rows + 63) / 64 * st.n2 * 4;
         if (!k) ctl = reinterpret_cast<uint32_t *>(w);
-        w += 256;
+        w += 256;  // one control block carved per item: fk20_rows_workspace_bytes reserves two
         it.table = st.table;
         it.inf = st.inf;
         it.rows = items[k].rows;
@@ -457,6 +487,45 @@ static hipError_t enqueue_items(fk20_ctx *c, const fk20_batch_item_t *items, uns
         it.stage = -1;
     }
     return launch_batch(L, c->T, c->prog.ops, c->prog.index, s, hook, c->use_chain ? ctl : nullptr);
+}
+
+static hipError_t enqueue_items(fk20_ctx *c, const fk20_batch_item_t *items, unsigned n, hipStream_t s,
+                                LaunchHook hook, void *workspace = nullptr) {
+    char *w = reinterpret_cast<char *>(workspace ? workspace : c->work);
+    unsigned groups = 0;
+    for (unsigned k = 0; k < n; k++) groups += (items[k].rows + 63) / 64;
+    // the context path with at least two 64-row groups, launches not timed one by one
+    if (workspace || c->nsub < 2 || groups < 2 || hook.fn) return enqueue_launch(c, items, n, s, hook, w);
+    // halves: each item's rows split at a 64-row boundary (items of one group go to the first)
+    fk20_batch_item_t half[2][MAX_ITEMS];
+    unsigned nh[2] = {0, 0};
+    for (unsigned k = 0; k < n; k++) {
+        const unsigned rg = (items[k].rows + 63) / 64, r0 = std::min(items[k].rows, (rg + 1) / 2 * 64);
+        const size_t pbytes = (size_t)c->setups[items[k].setup_id].N * 32;
+        const size_t obytes = (size_t)(c->setups[items[k].setup_id].N / 8) * fk20_output_point_bytes(c->out_format);
+        fk20_batch_item_t a = items[k], b = items[k];
+        a.rows = r0;
+        b.rows = items[k].rows - r0;
+        b.polynomial = reinterpret_cast<const fk20_fr_t *>(reinterpret_cast<const char *>(items[k].polynomial) + r0 * pbytes);
+        b.h_fft = reinterpret_cast<fk20_g1p_t *>(reinterpret_cast<char *>(items[k].h_fft) + r0 * obytes);
+        half[0][nh[0]++] = a;
+        if (b.rows) half[1][nh[1]++] = b;
+    }
+    if (!nh[1]) return enqueue_launch(c, items, n, s, hook, w);
+    // the halves' workspaces: a 64-row split keeps the flag words, and each half carves one
+    // 256-byte control block of the two the whole item reserves, so both fit in the whole's size
+    size_t off = 0;  // bytes enqueue_launch carves for the first half (256 below the reserved size per item)
+    for (unsigned k = 0; k < nh[0]; k++)
+        off += fk20_rows_workspace_bytes(half[0][k].rows, c->setups[half[0][k].setup_id].N) - 256;
+    hipError_t e = hipEventRecord(c->fork, s);
+    for (int h = 0; h < 2 && e == hipSuccess; h++) e = hipStreamWaitEvent(c->sub[h], c->fork, 0);
+    if (e == hipSuccess) e = enqueue_launch(c, half[0], nh[0], c->sub[0], hook, w);
+    if (e == hipSuccess) e = enqueue_launch(c, half[1], nh[1], c->sub[1], hook, w + off);
+    for (int h = 0; h < 2 && e == hipSuccess; h++) {
+        e = hipEventRecord(c->join[h], c->sub[h]);
+        if (e == hipSuccess) e = hipStreamWaitEvent(s, c->join[h], 0);
+    }
+    return e;
 }
 
 int fk20_prove_batch(fk20_ctx_t *c, const fk20_batch_item_t *items, unsigned nitems, fk20_stream_t stream) {

@@ -52,6 +52,7 @@ _SIGS = {
     "fk20_ctx_destroy": ([_P], _I),
     "fk20_ctx_set_graph": ([_P, _I], _I),
     "fk20_ctx_set_chain": ([_P, _I], _I),
+    "fk20_ctx_set_streams": ([_P, _U], _I),
     "fk20_ctx_set_output": ([_P, _I], _I),
     "fk20_ctx_get_output": ([_P, ctypes.POINTER(_I)], _I),
     "fk20_output_point_bytes": ([_I], _SZ),

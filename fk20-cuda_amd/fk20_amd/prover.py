@@ -140,6 +140,11 @@ class Prover:
         hb.free()
         return out
 
+    def set_streams(self, n):
+        """1: one launch sequence on the caller's stream; 2 (default): the rows split in two
+        halves on two internal streams forked from and joined to the caller's stream."""
+        call("fk20_ctx_set_streams", self.ctx, int(n))
+
     def set_chain(self, enable):
         """Transform stages as one persistent launch, or one launch per stage (default)."""
         call("fk20_ctx_set_chain", self.ctx, int(bool(enable)))

@@ -126,6 +126,13 @@ int fk20_ctx_set_graph(fk20_ctx_t *ctx, int enable);
  * one launch per stage.  Results are identical; on MI355X the per-stage launches are faster
  * (the persistent form pays agent-scope cache write-back/invalidate around every butterfly). */
 int fk20_ctx_set_chain(fk20_ctx_t *ctx, int enable);
+/* 2 (default): fk20_prove_rows / fk20_prove_batch split the rows into two halves at a 64-row
+ * boundary and run the halves' launch sequences on two internal streams forked from and
+ * joined back to the caller's stream (captured as such in a hipGraph), so that one half's
+ * stage launches fill the SIMDs the other half's leave idle at a launch boundary;
+ * 1: a single sequence on the caller's stream.  Results are identical.  fk20_prove_batch_ws
+ * (the serving loop's, which overlaps batches itself) always uses the caller's stream. */
+int fk20_ctx_set_streams(fk20_ctx_t *ctx, unsigned n);
 /* Output format of fk20_prove_rows / fk20_prove_batch (h_fft buffers), per context:
  *   FK20_OUT_G1P           g1p_t[rows][n2], natural order: the reference ABI (default)
  *   FK20_OUT_G1P_DA        g1p_t, data-availability order: FK20Py list_to_reverse_bit_order
@@ -151,7 +158,7 @@ int fk20_setup_load(fk20_ctx_t *ctx, const fk20_g1p_t *xext_fft, unsigned N, uns
                     int *setup_id, fk20_stream_t stream);
 int fk20_setup_unload(fk20_ctx_t *ctx, int setup_id);
 /* Workspace bytes fk20_prove_rows needs for `rows` rows of length N:
- * rows * (N/8) * 656 + ceil(rows/64) * (N/8) * 4 + 256. */
+ * rows * (N/8) * 656 + ceil(rows/64) * (N/8) * 4 + 512. */
 size_t fk20_rows_workspace_bytes(unsigned rows, unsigned N);
 /* Grows the context workspace to fit `rows` rows of length N. */
 int fk20_ctx_reserve(fk20_ctx_t *ctx, unsigned rows, unsigned N);

@@ -303,3 +303,38 @@ def g1_decompress(b):
     if (y > P_MOD - y) != bool(b[0] & 0x20):
         y = P_MOD - y
     return x, y
+
+
+# ---- affine G1 helpers for the pairing-based verifier (oracle/pairing.py) ----
+
+def g1_affine(p):
+    """g1p (18,) -> (x, y) integers, or None for infinity."""
+    if g1p_is_inf(p):
+        return None
+    a = to_affine(p)
+    return limbs_to_int(a[:6]), limbs_to_int(a[6:])
+
+
+def g1p_from_affine(a):
+    return g1p_inf() if a is None else g1p_from_ints(a[0], a[1], 1)
+
+
+def g1_affine_add(a, b):
+    return g1_affine(g1p_op("add", g1p_from_affine(a), g1p_from_affine(b)))
+
+
+def g1_affine_mul(a, k):
+    return g1_affine(g1p_op("mul", g1p_from_affine(a), fr(k % R_MOD)))
+
+
+def fk20_proof_coset(k, N):
+    """Coset shift of h_fft[k] for polynomial length N: omega_{2N}^k (the proof covers the 16
+    points x * omega_16^i, the roots of X^16 - x^16)."""
+    return pow(pow(5, (R_MOD - 1) // (2 * N), R_MOD), k, R_MOD)
+
+
+def poly_eval(coeffs, x):
+    acc = 0
+    for c in reversed([limbs_to_int(v) for v in np.asarray(coeffs)]):
+        acc = (acc * x + c) % R_MOD
+    return acc

@@ -147,3 +147,22 @@ def test_chain_equals_per_stage_launches(prover):
         assert np.array_equal(aff(a[r]), aff(shadow_points(poly[r], tau)))
     for sid, _ in groups:
         prover.unload_setup(sid)
+
+
+def test_two_stream_split_equals_single_stream(prover):
+    """The default two-halves launch layout gives bit-identical outputs to one launch sequence,
+    for items of one, two and three 64-row groups in one batch, directly and from a graph."""
+    groups = []
+    for N, rows, seed in ((2048, 40, 601), (1024, 128, 602), (4096, 130, 603)):
+        sid = prover.load_setup(O.xext_fft_from_tau(S.random_secret(seed), N), N, 8)
+        groups.append((sid, S.random_polys(rows, N, seed + 10)))
+    split = prover.prove_batch(groups)
+    try:
+        prover.set_streams(1)
+        single = prover.prove_batch(groups)
+    finally:
+        prover.set_streams(2)
+    for a, b in zip(split, single):
+        assert np.array_equal(a, b)
+    for sid, _ in groups:
+        prover.unload_setup(sid)

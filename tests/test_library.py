@@ -103,3 +103,10 @@ def test_commit_sizes_and_argument_checks():
     assert lib.fk20_commit_workspace_bytes(512, 4096) == (256 + 16) * 512 * 144
     assert lib.fk20_commit_setup_load(None, None, 4096, 8, None, None) == -1
     assert lib.fk20_commit_rows(None, 0, None, None, 1, None) == -1
+
+
+def test_context_setters_reject_null():
+    lib = L.lib()
+    assert lib.fk20_ctx_set_streams(None, 2) == -1
+    assert lib.fk20_ctx_set_chain(None, 1) == -1
+    assert lib.fk20_ctx_set_graph(None, 1) == -1

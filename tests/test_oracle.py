@@ -334,3 +334,41 @@ def test_commit_direct_equals_closed_form():
     assert O.g1p_eq(got[0], O.commit_closed_form(poly[0], tau))
     assert O.g1p_is_inf(got[1])
     assert O.g1p_eq(got[2], O.g1p_op("mul", srs[5], poly[2, 5]))
+
+
+def test_pairing_pins():
+    """The verifier's pairing: G2 generator on the twist with order r, e(G1, G2) != 1 with
+    e^r = 1, and bilinearity e(aG1, bG2) = e(G1, G2)^(ab) = e(abG1, G2)."""
+    import pairing as PR
+    G2 = PR.G2_GEN
+    assert PR.g2_on_curve(G2) and PR.g2_mul(G2, PR.R) is None and PR.g2_mul(G2, PR.R - 1) == PR.g2_neg(G2)
+    g1 = O.g1_affine(O.g1p_gen())
+    e = PR.pairing(g1, G2)
+    assert e != PR.F12_ONE
+    assert PR.f12_pow(e, PR.R) == PR.F12_ONE
+    a, b = 0x1234567890ABCDEF1234567, 0xFEDCBA9876543210FEDC
+    lhs = PR.pairing(O.g1_affine_mul(g1, a), PR.g2_mul(G2, b))
+    assert lhs == PR.f12_pow(e, a * b % PR.R)
+    assert PR.pairing_product_is_one([(O.g1_affine_mul(g1, a * b), G2), (O.g1_affine_mul(g1, PR.R - 1), PR.g2_mul(G2, a * b))])
+
+
+def test_check_proof_multi_on_oracle_proofs():
+    """kzg_proofs.py:207 check on proofs from the oracle's FK20 (N = 1024): a proof verifies
+    against the commitment and the true evaluations, and fails with one evaluation changed."""
+    import pairing as PR
+    N = 1024
+    tau = S.random_secret(71)
+    poly = S.random_polys(1, N, 72)
+    srs = O.setup_srs(tau, N)
+    proofs = O.poly2h_fft(poly, O.xext_fft_from_tau(tau, N))
+    commitment = O.g1_affine(O.commit(poly, srs)[0])
+    srs_aff = [O.g1_affine(srs[i]) for i in range(16)]
+    g2_s16 = PR.g2_mul(PR.G2_GEN, pow(tau, 16, PR.R))
+    w16 = pow(5, (PR.R - 1) // 16, PR.R)
+    for k in (3, 100):
+        x = O.fk20_proof_coset(k, N)
+        ys = [O.poly_eval(poly[0], x * pow(w16, i, PR.R) % PR.R) for i in range(16)]
+        proof = O.g1_affine(proofs[0, k])
+        assert PR.check_proof_multi(commitment, proof, x, ys, srs_aff, g2_s16, O.g1_affine_add, O.g1_affine_mul)
+        ys[5] = (ys[5] + 1) % PR.R
+        assert not PR.check_proof_multi(commitment, proof, x, ys, srs_aff, g2_s16, O.g1_affine_add, O.g1_affine_mul)
