@@ -642,8 +642,13 @@ def main():
         "traffic_detail": None,
         "valu_counters": None,
         "work": "executed Fp muls of the kernel's algorithm x 300 mul32 (12x32-bit CIOS convention)",
+        "timing": "per-launch HIP events on the launch stream, one launch sequence of all rows "
+                  "(the --streams 1 layout: each launch has the GPU to itself)",
         "fp_mul_per_s": round(exe_fp / (avg_ms / 1e3), 1),
         "achieved_canonical": round(canon / (avg_ms / 1e3) / 1e12, 4),
+        # the whole step in the benchmarked layout: executed work of every kernel / step time
+        "achieved_step": round(WL.executed_row_fp_muls(args.N, args.window) * args.rows * WL.FP_MUL32
+                               / (ms / 1e3) / 1e12, 4),
     }
 
     roofline["valu_counters"] = measured_valu(dom, args.N, args.rows, args.window, avg_ms / 1e3)
