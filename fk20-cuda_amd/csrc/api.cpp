@@ -48,6 +48,25 @@ std::vector<std::unique_ptr<DeviceState>> g_dev;
 
 }  // namespace
 
+void fk20::glv_split(const uint32_t *k, uint32_t *out) {
+    using u128 = unsigned __int128;
+    const u128 lam = ((u128)GLV_LAMBDA[1] << 64) | GLV_LAMBDA[0];
+    u128 r = 0, q = 0;
+    for (int i = 255; i >= 0; i--) {  // long division by lambda (> 2^127), one bit at a time
+        const bool top = (r >> 127) != 0;
+        r = (r << 1) | ((k[i / 32] >> (i % 32)) & 1u);
+        q <<= 1;
+        if (top || r >= lam) {
+            r -= lam;
+            q |= 1;
+        }
+    }
+    for (int i = 0; i < 4; i++) {
+        out[i] = (uint32_t)(r >> (32 * i));
+        out[4 + i] = (uint32_t)(q >> (32 * i));
+    }
+}
+
 void fk20::build_host_tables(DevTables &T) {
     // w = 5^((r-1)/512) in Montgomery form, by square-and-multiply on the host copy of
     // the device Fr arithmetic
@@ -96,6 +115,8 @@ void fk20::build_host_tables(DevTables &T) {
         memcpy(T.inv2k_std[j], std.l, 32);
         fs_mul(acc, acc, half);
     }
+    for (int i = 0; i <= 512; i++) glv_split(T.tw_std[i], T.tw_glv[i]);
+    for (int j = 0; j < 10; j++) glv_split(T.inv2k_std[j], T.inv2k_glv[j]);
 }
 
 namespace {

@@ -14,6 +14,8 @@ struct DevTables {
     uint32_t tw_std[513][8];      // w^i, standard form (scalar-multiplier input)
     uint32_t inv2k_mont[10][8];   // 2^-j mod r, Montgomery form, j = 0..9
     uint32_t inv2k_std[10][8];    // 2^-j mod r, standard form
+    uint32_t tw_glv[513][8];      // w^i = k1 + k2 lambda: k1 (words 0..3), k2 (words 4..7)
+    uint32_t inv2k_glv[10][8];    // 2^-j the same way
 };
 
 // Reference ABI element sizes in 32-bit words
@@ -33,6 +35,10 @@ __host__ __device__ __forceinline__ unsigned bitrev(unsigned x, unsigned bits) {
 
 // Host copy of the constant tables (api.cpp).
 void build_host_tables(DevTables &T);
+
+// GLV split of a standard-form scalar k < 2^255 (8 words): k = k2 lambda + k1 with k1, k2 <
+// 2^128, as out[0..3] = k1 and out[4..7] = k2 (api.cpp).
+void glv_split(const uint32_t *k, uint32_t *out);
 
 // FK20_* status of a HIP error, recorded as the thread's last HIP error (api.cpp).
 int status_from_hip(hipError_t e);

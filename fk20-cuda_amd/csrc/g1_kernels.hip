@@ -6,8 +6,9 @@
 // points in LDS (packed Montgomery form, 144 B per point, 72 KiB for n = 512), one
 // butterfly per thread and stage: the structure of g1p_fft.cu:25-170 (forward) and
 // g1p_fft.cu:179-330 (inverse), with the inverse written as a decimation-in-time over
-// w^-1 followed by the 1/n scaling.  Twiddle multiplication is double-and-add over the
-// standard-form root (g1p_mul.cu:18-66).
+// w^-1 followed by the 1/n scaling.  Twiddle multiplication uses the GLV halves of the root
+// (precomputed per root) with 2-bit joint windows -- 128 doublings and 64 additions instead
+// of the reference's 256-bit double-and-add (g1p_mul.cu:18-66).
 #include "common.hip.h"
 
 namespace fk20 {
@@ -56,7 +57,12 @@ __device__ __forceinline__ void g1_gstore_std(uint32_t *g, const g1p &p) {
         reinterpret_cast<uint4 *>(g)[i] = make_uint4(w[4 * i], w[4 * i + 1], w[4 * i + 2], w[4 * i + 3]);
 }
 
-// Scalar multiplication with a non-unrolled bit loop (one inlined dbl and add).
+// Twiddle and 1/n multiplications: GLV halves with 2-bit joint windows (one shared copy).
+__device__ __noinline__ void g1_mul_glv_dev(g1p &r, const g1p &p, const uint32_t *kk) {
+    g1_mul_glv(r, p, kk);
+}
+
+// Scalar multiplication with a non-unrolled bit loop (one inlined dbl and add); any 256-bit k.
 __device__ __noinline__ void g1_mul_dev(g1p &r, const g1p &p, const uint32_t *k) {
     g1p q;
     g1_inf(q);
@@ -78,7 +84,7 @@ __device__ __forceinline__ void g1_butterfly(uint32_t *lds, unsigned l, unsigned
                                              const DevTables *T) {
     g1p x, y, a;
     g1_lds_load(x, lds + r * 36);
-    if (idx != 0) g1_mul_dev(x, x, T->tw_std[INV ? 512 - idx : idx]);
+    if (idx != 0) g1_mul_glv_dev(x, x, T->tw_glv[INV ? 512 - idx : idx]);
     g1_lds_load(y, lds + l * 36);
     g1_add(a, y, x);
     g1_neg(x, x);
@@ -117,7 +123,7 @@ __global__ void __launch_bounds__(256) k_g1_fft(uint32_t *out, const uint32_t *i
     for (unsigned e = t; e < n; e += half) {
         g1p p;
         g1_lds_load(p, lds + e * 36);
-        if (INV) g1_mul_dev(p, p, T->inv2k_std[LOGN]);
+        if (INV) g1_mul_glv_dev(p, p, T->inv2k_glv[LOGN]);
         g1_gstore_std(out + (size_t)e * G1P_WORDS, p);
     }
 }
@@ -143,7 +149,7 @@ __global__ void __launch_bounds__(256) k_hext_fft2h_fft(uint32_t *h_fft, const u
     // bit-reversed slot for the forward DIT and put infinity in the odd slots
     g1p p;
     g1_lds_load(p, lds + t * 36);
-    g1_mul_dev(p, p, T->inv2k_std[LOGN]);
+    g1_mul_glv_dev(p, p, T->inv2k_glv[LOGN]);
     __syncthreads();
     const unsigned dst = bitrev(t, LOGN);
     g1_lds_store(lds + dst * 36, p);
@@ -178,35 +184,76 @@ __global__ void k_std_to_packed(uint32_t *packed, const uint32_t *pts, size_t co
     g1_lds_store(packed + e * 36, p);
 }
 
-// hext_fft[row][j] = sum_i tc_fft[row][i][j] * xext[i][j] (fk20_msm.cu:23-66), one thread per
-// (row, column).  The 16 scalar multiplications share one doubling chain (Straus).
-__global__ void __launch_bounds__(64) k_msm_plain(uint32_t *hext, const uint32_t *tc_fft,
-                                                  const uint32_t *xpk, unsigned n2) {
-    const unsigned j = blockIdx.x * 64 + threadIdx.x;
-    const size_t row = blockIdx.y;
-    if (j >= n2) return;
-    const uint32_t *tc = tc_fft + row * 16 * n2 * FR_WORDS;
+// hext_fft[row][j] = sum_i tc_fft[row][i][j] * xext[i][j] (fk20_msm.cu:23-66), for any 256-bit
+// scalars.  A workgroup is one column j for 64 rows (one row per lane), so the 16 bases are
+// shared: their multiples 1..8 X_i are built once into LDS (16 x 8 points, 18 KiB) and every
+// lane runs Straus over radix-16 Booth digits of its 16 scalars,
+//     d_w = b[4w-1] + b[4w] + 2 b[4w+1] + 4 b[4w+2] - 8 b[4w+3]  in [-8, 8],  b[-1] = 0,
+// so that sum_{w<64} d_w 16^w = k - b[255] 2^256: the top bit is one more window d_64 = b[255].
+// Every digit reads 5 scalar bits and no carries, so all lanes run the same steps: 64
+// windows of 4 doublings and 16 complete additions (digit 0 adds infinity).
+__device__ __forceinline__ void msm_cols_add(g1p &acc, const uint32_t *tab, int dg) {
+    const unsigned a = (unsigned)(dg < 0 ? -dg : dg);
+    g1p X;
+    g1_lds_load(X, tab + (a ? a - 1 : 0) * 36);
+    if (dg < 0) fe_neg<2>(X.y, X.y);
+    if (a == 0) g1_inf(X);
+    g1_add(acc, acc, X);
+}
+
+__global__ void __launch_bounds__(64, 2) k_msm_cols(uint32_t *hext, const uint32_t *tc_fft, const uint32_t *xpk,
+                                                 unsigned n2, unsigned rows) {
+    __shared__ __attribute__((aligned(16))) uint32_t lds[16 * 8 * 36];
+    __shared__ uint32_t kbuf[16 * 64];  // the current scalar word of each (base, lane)
+    const unsigned j = blockIdx.x, t = threadIdx.x;
+    {
+        // lane t builds d X_i and (d + 1) X_i, i = t / 4, d = 2 (t % 4) + 1; entry a - 1 holds a X_i
+        const unsigned i = t / 4, d = 2 * (t % 4) + 1;
+        g1p X, e;
+        g1_lds_load(X, xpk + ((size_t)i * n2 + j) * 36);  // packed Montgomery points in HBM
+        g1_inf(e);
+#pragma unroll 1
+        for (int b = 2; b >= 0; b--) {
+            g1_dbl(e, e);
+            if ((d >> b) & 1) g1_add(e, e, X);
+        }
+        g1_lds_store(lds + (i * 8 + d - 1) * 36, e);
+        g1_add(e, e, X);
+        g1_lds_store(lds + (i * 8 + d) * 36, e);
+    }
+    __syncthreads();
+    const unsigned row = min(blockIdx.y * 64 + t, rows - 1);  // lanes past the last row repeat it
+    const uint32_t *tc = tc_fft + ((size_t)row * 16 * n2 + j) * FR_WORDS;  // scalar i at tc[i n2 8]
+    const size_t istride = (size_t)n2 * FR_WORDS;
     g1p acc;
     g1_inf(acc);
 #pragma unroll 1
-    for (int w = 7; w >= 0; w--) {
-        uint32_t kw[16];
-#pragma unroll
-        for (int i = 0; i < 16; i++) kw[i] = tc[((size_t)i * n2 + j) * FR_WORDS + w];
+    for (unsigned i = 0; i < 16; i++) msm_cols_add(acc, lds + i * 8 * 36, (int)(tc[i * istride + 7] >> 31));
 #pragma unroll 1
-        for (int b = 31; b >= 0; b--) {
+    for (int wd = 7; wd >= 0; wd--) {
+        uint32_t low = 0;  // bit i: bit 32 wd - 1 of scalar i
+#pragma unroll
+        for (int i = 0; i < 16; i++) {
+            kbuf[i * 64 + t] = tc[i * istride + wd];  // each lane reads back only its own words
+            if (wd > 0) low |= (tc[i * istride + wd - 1] >> 31) << i;
+        }
+#pragma unroll 1
+        for (int q = 7; q >= 0; q--) {
+            g1_dbl(acc, acc);
+            g1_dbl(acc, acc);
+            g1_dbl(acc, acc);
             g1_dbl(acc, acc);
 #pragma unroll 1
-            for (int i = 0; i < 16; i++) {
-                if ((kw[i] >> b) & 1) {
-                    g1p X;
-                    g1_lds_load(X, xpk + ((size_t)i * n2 + j) * 36);
-                    g1_add(acc, acc, X);
-                }
+            for (unsigned i = 0; i < 16; i++) {
+                const uint32_t kw = kbuf[i * 64 + t];
+                const uint32_t bits = kw >> (4 * q);
+                const uint32_t below = q ? (kw >> (4 * q - 1)) & 1u : (low >> i) & 1u;
+                const int dg = (int)(bits & 7u) + (int)below - (int)(bits & 8u);
+                msm_cols_add(acc, lds + i * 8 * 36, dg);
             }
         }
     }
-    g1_gstore_std(hext + (row * n2 + j) * G1P_WORDS, acc);
+    if (blockIdx.y * 64 + t < rows) g1_gstore_std(hext + ((size_t)row * n2 + j) * G1P_WORDS, acc);
 }
 
 // xext[i][m] = srs[N-17-i-16m] for m <= k-2, infinity otherwise (FK20Py fk20_multi.py:82-85:
@@ -424,7 +471,7 @@ hipError_t launch_std_to_packed(uint32_t *packed, const uint32_t *pts, size_t co
 hipError_t launch_msm_plain(uint32_t *hext, const uint32_t *tc_fft, const uint32_t *xpk, unsigned rows,
                             unsigned n2, hipStream_t s) {
     if (!rows) return hipSuccess;
-    hipLaunchKernelGGL(k_msm_plain, dim3((n2 + 63) / 64, rows), dim3(64), 0, s, hext, tc_fft, xpk, n2);
+    hipLaunchKernelGGL(k_msm_cols, dim3(n2, (rows + 63) / 64), dim3(64), 0, s, hext, tc_fft, xpk, n2, rows);
     return hipGetLastError();
 }
 
