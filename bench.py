@@ -34,12 +34,7 @@ from fk20_amd.prover import LAUNCH_KINDS, Prover  # noqa: E402
 
 R_MOD = S.R_MOD
 METRIC = "FK20 rows proved/sec (N=4096, 512 rows) batched vs plain; p50/p99 latency, bit-exact"
-G1_GEN = np.zeros(18, dtype=np.uint64)
-G1_GEN[:6] = [0xFB3AF00ADB22C6BB, 0x6C55E83FF97A1AEF, 0xA14E3A3F171BAC58,
-              0xC3688C4F9774B905, 0x2695638C4FA9AC0F, 0x17F1D3A73197D794]
-G1_GEN[6:12] = [0x0CAA232946C5E7E1, 0xD03CC744A2888AE4, 0x00DB18CB2C04B3ED,
-                0xFCF5E095D5D00AF6, 0xA09E30ED741D8AE4, 0x08B3F481E3AAA0F1]
-G1_GEN[12] = 1
+G1_GEN = S.G1_GEN
 
 
 def log(msg):
@@ -109,22 +104,7 @@ def pct(v, q):
     return v[min(len(v) - 1, int(round(q / 100.0 * (len(v) - 1))))]
 
 
-def device_setup(N, tau):
-    """SRS [tau^i]G on the GPU, then xext_fft by fk20_setup2xext_fft (FK20Py toeplitz_part1)."""
-    pw, acc = [], 1
-    for _ in range(N + 1):
-        pw.append(S.int_to_fr(acc))
-        acc = acc * tau % R_MOD
-    gens = L.DeviceBuffer.from_array(np.tile(G1_GEN, (N + 1, 1)))
-    ks = L.DeviceBuffer.from_array(np.array(pw))
-    srs = L.DeviceBuffer((N + 1) * 144)
-    L.call("fk20_g1p_op", 3, srs.ptr, gens.ptr, None, ks.ptr, N + 1, None)
-    xext = L.DeviceBuffer(16 * (N // 8) * 144)
-    L.call("fk20_setup2xext_fft", xext.ptr, srs.ptr, N, None)
-    L.synchronize()
-    for b in (gens, ks, srs):
-        b.free()
-    return xext
+device_setup = S.device_setup
 
 
 class Batch:

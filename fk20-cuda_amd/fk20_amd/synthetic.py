@@ -81,3 +81,42 @@ def fr_to_int(a):
 
 def int_to_fr(v):
     return np.array([(v >> (64 * i)) & 0xFFFFFFFFFFFFFFFF for i in range(4)], dtype=np.uint64)
+
+
+# BLS12-381 G1 generator as a g1p_t (x, y, z = 1), standard form (FK20-CUDA g1p.cu:95-115)
+G1_GEN = np.zeros(18, dtype=np.uint64)
+G1_GEN[:6] = [0xFB3AF00ADB22C6BB, 0x6C55E83FF97A1AEF, 0xA14E3A3F171BAC58,
+              0xC3688C4F9774B905, 0x2695638C4FA9AC0F, 0x17F1D3A73197D794]
+G1_GEN[6:12] = [0x0CAA232946C5E7E1, 0xD03CC744A2888AE4, 0x00DB18CB2C04B3ED,
+                0xFCF5E095D5D00AF6, 0xA09E30ED741D8AE4, 0x08B3F481E3AAA0F1]
+G1_GEN[12] = 1
+
+
+def device_srs(count, tau):
+    """SRS [tau^i]G, i < count, computed on the GPU (fk20_g1p_op mul): a DeviceBuffer of
+    count g1p_t (FK20Py kzg_proofs.py:22-28 generate_setup, G1 half)."""
+    from . import lib as L
+    pw, acc = [], 1
+    for _ in range(count):
+        pw.append(int_to_fr(acc))
+        acc = acc * tau % R_MOD
+    gens = L.DeviceBuffer.from_array(np.tile(G1_GEN, (count, 1)))
+    ks = L.DeviceBuffer.from_array(np.array(pw))
+    srs = L.DeviceBuffer(count * 144)
+    L.call("fk20_g1p_op", 3, srs.ptr, gens.ptr, None, ks.ptr, count, None)
+    L.synchronize()
+    gens.free()
+    ks.free()
+    return srs
+
+
+def device_setup(N, tau):
+    """xext_fft (16, N/8) of the secret tau on the GPU: SRS [tau^i]G (i <= N), then
+    fk20_setup2xext_fft (FK20Py fk20_multi.py:82-85 toeplitz_part1).  A DeviceBuffer."""
+    from . import lib as L
+    srs = device_srs(N + 1, tau)
+    xext = L.DeviceBuffer(16 * (N // 8) * 144)
+    L.call("fk20_setup2xext_fft", xext.ptr, srs.ptr, N, None)
+    L.synchronize()
+    srs.free()
+    return xext
