@@ -10,6 +10,7 @@
 
 #include "../../include/fk20.h"
 #include "common.hip.h"
+#include "fast.hip.h"
 
 using namespace fk20;
 
@@ -33,12 +34,14 @@ hipStream_t S(fk20_stream_t s) { return reinterpret_cast<hipStream_t>(s); }
 
 bool valid_N(unsigned N) { return N == 1024 || N == 2048 || N == 4096; }
 bool valid_n(unsigned n) { return n == 128 || n == 256 || n == 512; }
+unsigned log2u(unsigned n) { return 31u - (unsigned)__builtin_clz(n); }
 
 // ---------------------------------------------------------------- per-device state
 
 struct DeviceState {
     DevTables *tables = nullptr;            // device copy
-    void *scratch = nullptr;                // fk20_poly2h_fft / fk20_msm scratch
+    Programs prog;                          // twiddle programs of the drop-in transforms
+    void *scratch = nullptr;                // workspace of the drop-in entry points
     size_t scratch_bytes = 0;
 };
 
@@ -47,25 +50,6 @@ std::mutex g_mu;
 std::vector<std::unique_ptr<DeviceState>> g_dev;
 
 }  // namespace
-
-void fk20::glv_split(const uint32_t *k, uint32_t *out) {
-    using u128 = unsigned __int128;
-    const u128 lam = ((u128)GLV_LAMBDA[1] << 64) | GLV_LAMBDA[0];
-    u128 r = 0, q = 0;
-    for (int i = 255; i >= 0; i--) {  // long division by lambda (> 2^127), one bit at a time
-        const bool top = (r >> 127) != 0;
-        r = (r << 1) | ((k[i / 32] >> (i % 32)) & 1u);
-        q <<= 1;
-        if (top || r >= lam) {
-            r -= lam;
-            q |= 1;
-        }
-    }
-    for (int i = 0; i < 4; i++) {
-        out[i] = (uint32_t)(r >> (32 * i));
-        out[4 + i] = (uint32_t)(q >> (32 * i));
-    }
-}
 
 void fk20::build_host_tables(DevTables &T) {
     // w = 5^((r-1)/512) in Montgomery form, by square-and-multiply on the host copy of
@@ -115,8 +99,6 @@ void fk20::build_host_tables(DevTables &T) {
         memcpy(T.inv2k_std[j], std.l, 32);
         fs_mul(acc, acc, half);
     }
-    for (int i = 0; i <= 512; i++) glv_split(T.tw_std[i], T.tw_glv[i]);
-    for (int j = 0; j < 10; j++) glv_split(T.inv2k_std[j], T.inv2k_glv[j]);
 }
 
 namespace {
@@ -141,6 +123,7 @@ int device_state(DeviceState **out) {
         }
         st.tables = reinterpret_cast<DevTables *>(d);
     }
+    if (!st.prog.ops) FK20_TRY(build_programs(st.prog));
     *out = &st;
     return FK20_OK;
 }
@@ -230,13 +213,20 @@ int fr_ift(fk20_fr_t *output, const fk20_fr_t *input, unsigned count, unsigned n
     return fr_transform(output, input, count, n, true, stream);
 }
 
+// The G1 transforms run column-batched (lanes = transforms) through D in scratch, so the
+// output may alias the input as in the reference (g1p_fft.cu:17).
 static int g1_transform(fk20_g1p_t *out, const fk20_g1p_t *in, unsigned count, unsigned n, bool inv,
                         fk20_stream_t stream) {
     if (!valid_n(n) || (count && (!out || !in))) return FK20_EINVAL;
+    if (!count) return FK20_OK;
     DeviceState *st;
     int rc = device_state(&st);
     if (rc) return rc;
-    return hip_status(launch_g1_fft(W(out), W(in), count, n, inv, st->tables, S(stream)));
+    void *D;
+    rc = device_scratch(st, (size_t)count * n * 144, &D);
+    if (rc) return rc;
+    return hip_status(launch_plain(inv ? PLAIN_G1_IFT : PLAIN_G1_FFT, W(out), W(in), nullptr, count, log2u(n),
+                                   reinterpret_cast<uint32_t *>(D), nullptr, st->tables, st->prog, S(stream)));
 }
 
 int g1p_fft(fk20_g1p_t *output, const fk20_g1p_t *input, unsigned count, unsigned n, fk20_stream_t stream) {
@@ -276,13 +266,20 @@ int fk20_hext2h(fk20_g1p_t *h, unsigned rows, unsigned n2, fk20_stream_t stream)
 int fk20_hext_fft2h_fft(fk20_g1p_t *h_fft, const fk20_g1p_t *hext_fft, unsigned rows, unsigned n2,
                         fk20_stream_t stream) {
     if (!valid_n(n2) || (rows && (!h_fft || !hext_fft))) return FK20_EINVAL;
+    if (!rows) return FK20_OK;
     DeviceState *st;
     int rc = device_state(&st);
     if (rc) return rc;
-    return hip_status(launch_hext_fft2h_fft(W(h_fft), W(hext_fft), rows, n2, st->tables, S(stream)));
+    void *D;
+    rc = device_scratch(st, (size_t)rows * n2 * 144, &D);
+    if (rc) return rc;
+    return hip_status(launch_plain(PLAIN_HEXT_FFT2H_FFT, W(h_fft), W(hext_fft), nullptr, rows, log2u(n2),
+                                   reinterpret_cast<uint32_t *>(D), nullptr, st->tables, st->prog, S(stream)));
 }
 
-// fk20_poly2h_fft.cu:47-80: toeplitz -> NTT (16 per row) -> MSM -> IFFT -> zero -> FFT.
+// fk20_poly2h_fft.cu:47-80: toeplitz -> NTT (16 per row) -> MSM -> IFFT -> zero -> FFT, with
+// the 1/n2 of the IFFT moved onto the NTT output (a field multiplication instead of a
+// scalar multiplication per point) and the transforms column-batched over the rows.
 int fk20_poly2h_fft(fk20_g1p_t *h_fft, const fk20_fr_t *poly, const fk20_g1p_t *xext_fft, unsigned rows,
                     unsigned N, fk20_stream_t stream) {
     if (!valid_N(N) || (rows && (!h_fft || !poly || !xext_fft))) return FK20_EINVAL;
@@ -292,25 +289,19 @@ int fk20_poly2h_fft(fk20_g1p_t *h_fft, const fk20_fr_t *poly, const fk20_g1p_t *
     int rc = device_state(&st);
     if (rc) return rc;
     const size_t tc_bytes = (size_t)rows * 16 * n2 * 32;
-    const size_t h_bytes = (size_t)rows * n2 * 144;
+    const size_t d_bytes = (size_t)rows * n2 * 144;
     const size_t x_bytes = (size_t)16 * n2 * 144;
     void *scratch;
-    rc = device_scratch(st, tc_bytes + h_bytes + x_bytes, &scratch);
+    rc = device_scratch(st, tc_bytes + d_bytes + x_bytes, &scratch);
     if (rc) return rc;
     char *base = reinterpret_cast<char *>(scratch);
-    uint32_t *tc = reinterpret_cast<uint32_t *>(base);
-    uint32_t *h = reinterpret_cast<uint32_t *>(base + tc_bytes);
-    uint32_t *xpk = reinterpret_cast<uint32_t *>(base + tc_bytes + h_bytes);
+    uint32_t *tcT = reinterpret_cast<uint32_t *>(base);
+    uint32_t *D = reinterpret_cast<uint32_t *>(base + tc_bytes);
+    uint32_t *xpk = reinterpret_cast<uint32_t *>(base + tc_bytes + d_bytes);
     hipStream_t s = S(stream);
-    FK20_TRY(launch_toeplitz(tc, W(poly), rows, N, s));
-    FK20_TRY(launch_fr_ntt(tc, tc, rows * 16, n2, false, st->tables, s));
-    rc = msm_with_scratch(reinterpret_cast<fk20_g1p_t *>(h), reinterpret_cast<fk20_fr_t *>(tc), xext_fft, rows,
-                          n2, xpk, s);
-    if (rc) return rc;
-    FK20_TRY(launch_g1_fft(h, h, rows, n2, true, st->tables, s));
-    FK20_TRY(launch_hext2h(h, rows, n2, s));
-    FK20_TRY(launch_g1_fft(W(h_fft), h, rows, n2, false, st->tables, s));
-    return FK20_OK;
+    FK20_TRY(launch_std_to_packed(xpk, W(xext_fft), (size_t)16 * n2, s));
+    return hip_status(launch_plain(PLAIN_POLY2H_FFT, W(h_fft), W(poly), xpk, rows, log2u(n2), D, tcT, st->tables,
+                                   st->prog, s));
 }
 
 int fk20_setup2xext_fft(fk20_g1p_t *xext_fft, const fk20_g1p_t *setup, unsigned N, fk20_stream_t stream) {
@@ -318,9 +309,13 @@ int fk20_setup2xext_fft(fk20_g1p_t *xext_fft, const fk20_g1p_t *setup, unsigned 
     DeviceState *st;
     int rc = device_state(&st);
     if (rc) return rc;
+    void *D;
+    rc = device_scratch(st, (size_t)16 * (N / 8) * 144, &D);
+    if (rc) return rc;
     hipStream_t s = S(stream);
     FK20_TRY(launch_setup_gather(W(xext_fft), W(setup), N, s));
-    return hip_status(launch_g1_fft(W(xext_fft), W(xext_fft), 16, N / 8, false, st->tables, s));
+    return hip_status(launch_plain(PLAIN_G1_FFT, W(xext_fft), W(xext_fft), nullptr, 16, log2u(N / 8),
+                                   reinterpret_cast<uint32_t *>(D), nullptr, st->tables, st->prog, s));
 }
 
 // ---------------------------------------------------------------- helpers

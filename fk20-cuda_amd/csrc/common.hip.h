@@ -14,8 +14,6 @@ struct DevTables {
     uint32_t tw_std[513][8];      // w^i, standard form (scalar-multiplier input)
     uint32_t inv2k_mont[10][8];   // 2^-j mod r, Montgomery form, j = 0..9
     uint32_t inv2k_std[10][8];    // 2^-j mod r, standard form
-    uint32_t tw_glv[513][8];      // w^i = k1 + k2 lambda: k1 (words 0..3), k2 (words 4..7)
-    uint32_t inv2k_glv[10][8];    // 2^-j the same way
 };
 
 // Reference ABI element sizes in 32-bit words
@@ -36,10 +34,6 @@ __host__ __device__ __forceinline__ unsigned bitrev(unsigned x, unsigned bits) {
 // Host copy of the constant tables (api.cpp).
 void build_host_tables(DevTables &T);
 
-// GLV split of a standard-form scalar k < 2^255 (8 words): k = k2 lambda + k1 with k1, k2 <
-// 2^128, as out[0..3] = k1 and out[4..7] = k2 (api.cpp).
-void glv_split(const uint32_t *k, uint32_t *out);
-
 // FK20_* status of a HIP error, recorded as the thread's last HIP error (api.cpp).
 int status_from_hip(hipError_t e);
 
@@ -53,14 +47,14 @@ hipError_t launch_toeplitz_fft(uint32_t *tc_fft, const uint32_t *poly, unsigned 
                                const DevTables *T, hipStream_t s);
 hipError_t launch_fr_ntt(uint32_t *out, const uint32_t *in, unsigned count, unsigned n, bool inverse,
                          const DevTables *T, hipStream_t s);
-hipError_t launch_g1_fft(uint32_t *out, const uint32_t *in, unsigned count, unsigned n, bool inverse,
-                         const DevTables *T, hipStream_t s);
 hipError_t launch_hext2h(uint32_t *h, unsigned rows, unsigned n2, hipStream_t s);
-hipError_t launch_hext_fft2h_fft(uint32_t *h_fft, const uint32_t *hext_fft, unsigned rows, unsigned n2,
-                                 const DevTables *T, hipStream_t s);
 hipError_t launch_std_to_packed(uint32_t *packed, const uint32_t *std_pts, size_t count, hipStream_t s);
 hipError_t launch_msm_plain(uint32_t *hext_fft, const uint32_t *tc_fft, const uint32_t *xext_packed,
                             unsigned rows, unsigned n2, hipStream_t s);
+// The same MSM from transposed scalars tcT[j][i][row] into D[bitrev(j)][row] (packed
+// Montgomery), the layouts of the column-batched transforms.
+hipError_t launch_msm_plain_T(uint32_t *D, const uint32_t *tcT, const uint32_t *xext_packed, unsigned rows,
+                              unsigned n2, hipStream_t s);
 hipError_t launch_setup_gather(uint32_t *xext, const uint32_t *srs, unsigned N, hipStream_t s);
 hipError_t launch_to_affine(uint32_t *out, const uint32_t *in, size_t count, hipStream_t s);
 hipError_t launch_compress(uint32_t *out, const uint32_t *in, size_t count, hipStream_t s);

@@ -101,11 +101,6 @@ bool check_program(const uint32_t *ops, const fs &expect_mont) {
     return memcmp(acc.l, expect_mont.l, 32) == 0;
 }
 
-struct Programs {
-    uint32_t *ops = nullptr;    // device
-    uint32_t *index = nullptr;  // device, [2][512]
-};
-
 // Builds the programs of w^idx and w^-idx (w of order 512) and checks each on the host.
 bool build_programs_host(std::vector<uint32_t> &ops, std::vector<uint32_t> &index) {
     DevTables T;
@@ -125,16 +120,6 @@ bool build_programs_host(std::vector<uint32_t> &ops, std::vector<uint32_t> &inde
     return true;
 }
 
-hipError_t build_programs(Programs &P) {
-    std::vector<uint32_t> ops, index;
-    if (!build_programs_host(ops, index)) return hipErrorInvalidValue;
-    hipError_t err = hipMalloc(&P.ops, ops.size() * 4);
-    if (err == hipSuccess) err = hipMalloc(&P.index, index.size() * 4);
-    if (err == hipSuccess) err = hipMemcpy(P.ops, ops.data(), ops.size() * 4, hipMemcpyHostToDevice);
-    if (err == hipSuccess) err = hipMemcpy(P.index, index.data(), index.size() * 4, hipMemcpyHostToDevice);
-    return err;
-}
-
 // ---------------------------------------------------------------- context
 
 struct Setup {
@@ -152,6 +137,33 @@ unsigned nwin_of(unsigned w) { return (256 + w - 1) / w; }
 int status_of(hipError_t e) { return status_from_hip(e); }
 
 }  // namespace
+
+hipError_t fk20::build_programs(Programs &P) {
+    std::vector<uint32_t> ops, index;
+    if (!build_programs_host(ops, index)) return hipErrorInvalidValue;
+    DevTables T;
+    build_host_tables(T);
+    index.resize(TW_INDEX_SIZE);
+    for (int j = 0; j < 10; j++) {
+        index[TW_SCALE + j] = P.scale_off[j] = (uint32_t)ops.size();
+        append_program(ops, T.inv2k_std[j], TW_WNAF);
+        fs expect;
+        fs_set(expect, T.inv2k_mont[j]);
+        if (!check_program(ops.data() + index[TW_SCALE + j], expect)) return hipErrorInvalidValue;
+    }
+    hipError_t err = hipMalloc(&P.ops, ops.size() * 4);
+    if (err == hipSuccess) err = hipMalloc(&P.index, index.size() * 4);
+    if (err == hipSuccess) err = hipMemcpy(P.ops, ops.data(), ops.size() * 4, hipMemcpyHostToDevice);
+    if (err == hipSuccess) err = hipMemcpy(P.index, index.data(), index.size() * 4, hipMemcpyHostToDevice);
+    if (err != hipSuccess) free_programs(P);
+    return err;
+}
+
+void fk20::free_programs(Programs &P) {
+    if (P.ops) (void)hipFree(P.ops);
+    if (P.index) (void)hipFree(P.index);
+    P.ops = P.index = nullptr;
+}
 
 struct fk20_ctx {
     int device = 0;
@@ -233,8 +245,7 @@ int fk20_ctx_destroy(fk20_ctx_t *c) {
         if (s.inf) (void)hipFree(s.inf);
     }
     if (c->work) (void)hipFree(c->work);
-    if (c->prog.ops) (void)hipFree(c->prog.ops);
-    if (c->prog.index) (void)hipFree(c->prog.index);
+    free_programs(c->prog);
     if (c->gexec) (void)hipGraphExecDestroy(c->gexec);
     for (hipEvent_t e : c->events) (void)hipEventDestroy(e);
     for (int k = 0; k < 2; k++) {

@@ -27,6 +27,21 @@ static constexpr int TW_WNAF = FK20_TW_WNAF;
 // direction rows of the program index: index[dir * 512 + idx] = offset of w^(+-idx)'s program
 static constexpr int TW_FWD = 0;
 static constexpr int TW_INV = 1;
+// index[TW_SCALE + j] = offset of the program of 2^-j (j < 10), the 1/n of the inverse transforms
+static constexpr int TW_SCALE = 2 * 512;
+static constexpr int TW_INDEX_SIZE = TW_SCALE + 10;
+
+// Twiddle programs of one device (ctx.cpp): ops and index in device memory; the scale
+// programs' offsets also on the host, so launchers can pass them as plain pointers.
+struct Programs {
+    uint32_t *ops = nullptr;
+    uint32_t *index = nullptr;
+    uint32_t scale_off[10] = {};
+};
+// Builds and checks every program on the host (each evaluates to its scalar over Fr with
+// phi = lambda), then copies them to the current device.
+hipError_t build_programs(Programs &P);
+void free_programs(Programs &P);
 
 hipError_t launch_scale_bases(uint32_t *xs, uint32_t *inf, const uint32_t *xext, size_t count,
                               const uint32_t *scale, hipStream_t s);
@@ -48,6 +63,7 @@ struct Item {
     const uint32_t *inf;    // setup infinity flags
     uint32_t rows, N, logn2, win;
     uint32_t out_format;    // FK20_OUT_* (bit 0: data-availability order, bit 1: compressed)
+    uint32_t scale_tc;      // 1: the NTT output is scaled by 1/n2 (setups without folded 1/n2)
     int32_t stage;          // set per launch by launch_batch
 };
 struct Launch {
@@ -86,5 +102,19 @@ struct LaunchHook {
 // counter lives at `chain` (two words, cleared by the launch sequence itself).
 hipError_t launch_batch(const Launch &L, const DevTables *T, const uint32_t *ops, const uint32_t *index,
                         hipStream_t s, LaunchHook hook, uint32_t *chain = nullptr);
+
+// The drop-in entry points on the same column-batched transform kernels, for `rows`
+// transforms of n = 2^logn points (n2 = n for the FK20 kinds), with D = rows * n * 144
+// bytes of workspace (and tcT = rows * 16 * n * 32 for PLAIN_POLY2H_FFT):
+//   PLAIN_G1_FFT          out[r] = DFT(in[r])                         (g1p_fft)
+//   PLAIN_G1_IFT          out[r] = DFT^-1(in[r]), scaled by 1/n       (g1p_ift)
+//   PLAIN_HEXT_FFT2H_FFT  IFFT, zero upper half, FFT                  (fk20_hext_fft2h_fft)
+//   PLAIN_POLY2H_FFT      Toeplitz + NTT (scaled by 1/n2), MSM over xpk (16 x n2 packed
+//                         Montgomery bases), then as above            (fk20_poly2h_fft)
+// out may alias in: every input is read into D before the first output is written.
+enum PlainKind { PLAIN_G1_FFT = 0, PLAIN_G1_IFT = 1, PLAIN_HEXT_FFT2H_FFT = 2, PLAIN_POLY2H_FFT = 3 };
+hipError_t launch_plain(PlainKind kind, uint32_t *out, const uint32_t *in, const uint32_t *xpk, unsigned rows,
+                        unsigned logn, uint32_t *D, uint32_t *tcT, const DevTables *T, const Programs &P,
+                        hipStream_t s);
 
 }  // namespace fk20

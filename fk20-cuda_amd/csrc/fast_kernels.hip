@@ -223,9 +223,12 @@ __global__ void __launch_bounds__(256) k_toeplitz_fft_multi(Launch L, const DevT
         }
         __syncthreads();
     }
+    fs scale;
+    fs_set(scale, T->inv2k_mont[logn]);
     if (t < half) {
         for (unsigned e = t; e < n; e += half) {
             fs a = lds[e];
+            if (it.scale_tc) fs_mul(a, a, scale);
             fs_from_mont(a, a);
             st_words(it.tcT + (((size_t)e * 16 + i) * it.rows + row) * FR_WORDS, a.l, 2);
         }
@@ -595,6 +598,24 @@ __global__ void __launch_bounds__(64) k_output_multi(Launch L) {
     }
 }
 
+// ---------------------------------------------------------------- drop-in entry points
+
+// D[pos][row] = in[row][j] (standard form -> packed Montgomery), pos = bitrev(j) when brev,
+// times the scalar of the program at scale_ops when given.
+template <int W>
+__global__ void __launch_bounds__(64, 2) k_plain_load(uint32_t *D, const uint32_t *in, unsigned rows, unsigned logn,
+                                                     unsigned brev, const uint32_t *scale_ops) {
+    const unsigned rg = (rows + 63) / 64, j = blockIdx.x / rg;
+    const unsigned row = (blockIdx.x % rg) * 64 + threadIdx.x;
+    if (row >= rows) return;
+    uint32_t w[36];
+    ld_words(w, in + ((size_t)row * (1u << logn) + j) * G1P_WORDS, 9);
+    g1p p;
+    g1_load_std(p, w);
+    if (scale_ops) tw_mul<W>(p, scale_ops);
+    pt_store(D + ((size_t)(brev ? bitrev(j, logn) : j) * rows + row) * 36, p);
+}
+
 // ---------------------------------------------------------------- launchers
 
 hipError_t launch_scale_bases(uint32_t *xs, uint32_t *inf, const uint32_t *xext, size_t count,
@@ -736,6 +757,66 @@ hipError_t launch_batch(const Launch &base, const DevTables *T, const uint32_t *
     total = set_blocks(L, per);
     if (total) hipLaunchKernelGGL(k_output_multi, dim3(total), dim3(64), 0, s, L);
     hook(LK_OUTPUT);
+    return hipGetLastError();
+}
+
+// The drop-in transforms as one item of the batched launches: the DIT stage kernel runs
+// IFFT stages 0 .. L-1 (the last one through the middle's positions), the DIF kernel FFT
+// stages 0 .. L-1 (the first likewise), and the output kernel writes natural order from the
+// DIF's bit-reversed positions (format 0) or position by position (format 1).
+hipError_t launch_plain(PlainKind kind, uint32_t *out, const uint32_t *in, const uint32_t *xpk, unsigned rows,
+                        unsigned logn, uint32_t *D, uint32_t *tcT, const DevTables *T, const Programs &P,
+                        hipStream_t s) {
+    if (!rows) return hipSuccess;
+    Launch L;
+    memset(&L, 0, sizeof L);
+    L.nitems = 1;
+    Item &it = L.item[0];
+    it.poly = in;
+    it.h_fft = out;
+    it.tcT = tcT;
+    it.D = D;
+    it.rows = rows;
+    it.N = 8u << logn;
+    it.logn2 = logn;
+    it.scale_tc = 1;
+    const unsigned rg = (rows + 63) / 64, n = 1u << logn;
+    unsigned per[MAX_ITEMS] = {};
+    const uint32_t *scale = P.ops + P.scale_off[logn];
+    auto stage = [&](void (*kernel)(Launch, const uint32_t *, const uint32_t *), int st) {
+        it.stage = st;
+        per[0] = n / 2 * rg;
+        const unsigned total = set_blocks(L, per);
+        hipLaunchKernelGGL(kernel, dim3(total), dim3(64), 0, s, L, P.ops, P.index);
+    };
+    switch (kind) {
+    case PLAIN_G1_FFT:
+        hipLaunchKernelGGL(k_plain_load<TW_WNAF>, dim3(n * rg), dim3(64), 0, s, D, in, rows, logn, 0u,
+                           (const uint32_t *)nullptr);
+        for (unsigned t = 0; t < logn; t++) stage(k_dif_multi<TW_WNAF>, (int)t);
+        break;
+    case PLAIN_G1_IFT:
+        hipLaunchKernelGGL(k_plain_load<TW_WNAF>, dim3(n * rg), dim3(64), 0, s, D, in, rows, logn, 1u, scale);
+        for (unsigned t = 0; t < logn; t++) stage(k_dit_multi<TW_WNAF>, (int)t);
+        it.out_format = 1;
+        break;
+    case PLAIN_HEXT_FFT2H_FFT:
+    case PLAIN_POLY2H_FFT:
+        if (kind == PLAIN_HEXT_FFT2H_FFT) {
+            hipLaunchKernelGGL(k_plain_load<TW_WNAF>, dim3(n * rg), dim3(64), 0, s, D, in, rows, logn, 1u, scale);
+        } else {
+            per[0] = rows * 16;
+            hipLaunchKernelGGL(k_toeplitz_fft_multi, dim3(set_blocks(L, per)), dim3(256), 0, s, L, T);
+            hipError_t e = launch_msm_plain_T(D, tcT, xpk, rows, n, s);
+            if (e != hipSuccess) return e;
+        }
+        for (unsigned t = 0; t + 1 < logn; t++) stage(k_dit_multi<TW_WNAF>, (int)t);
+        stage(k_middle_multi<TW_WNAF>, 0);
+        for (unsigned t = 1; t < logn; t++) stage(k_dif_multi<TW_WNAF>, (int)t);
+        break;
+    }
+    per[0] = n * rg;
+    hipLaunchKernelGGL(k_output_multi, dim3(set_blocks(L, per)), dim3(64), 0, s, L);
     return hipGetLastError();
 }
 

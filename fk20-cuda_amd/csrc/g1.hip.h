@@ -185,38 +185,6 @@ FK20_HD void g1_mul(g1p &r, const g1p &p, const uint32_t *k) {
     r = q;
 }
 
-// r = (k1 + k2 lambda) p with phi(P) = (beta x, y, z) = lambda P, for k1, k2 < 2^128 given as
-// kk[0..3] (k1) and kk[4..7] (k2), little-endian words: 2-bit joint windows over a table
-// T[a + 4b] = a P + b phi(P) (a, b < 4, T[0] = infinity) in the caller's memory, so every
-// lane runs the same 64 steps of two doublings and one complete addition whatever its
-// scalar (the plain transforms' lanes each carry a different twiddle).  Valid for P of
-// order r (the subgroup every FK20 point lies in).
-FK20_HD void g1_mul_glv(g1p &r, const g1p &p, const uint32_t *kk) {
-    g1p T[16];
-    fe beta;
-    fe_set(beta, FP_BETA);
-    g1_inf(T[0]);
-    T[1] = p;
-    g1_dbl(T[2], p);
-    g1_add(T[3], T[2], p);
-    for (int a = 1; a < 4; a++) {
-        T[4 * a] = T[a];
-        fe_mul(T[4 * a].x, T[a].x, beta);
-    }
-    for (int b = 1; b < 4; b++)
-        for (int a = 1; a < 4; a++) g1_add(T[a + 4 * b], T[a], T[4 * b]);
-    g1p q;
-    g1_inf(q);
-    for (int i = 63; i >= 0; i--) {
-        g1_dbl(q, q);
-        g1_dbl(q, q);
-        const unsigned sh = (2 * i) % 32, w = (2 * i) / 32;
-        const unsigned sel = ((kk[w] >> sh) & 3u) | (((kk[4 + w] >> sh) & 3u) << 2);
-        g1_add(q, q, T[sel]);
-    }
-    r = q;
-}
-
 // Affine conversion: (x/z, y/z) canonical Montgomery, or the infinity flag when z = 0.
 FK20_HD bool g1_to_affine(fe &ax, fe &ay, const g1p &p) {
     fe zc;

@@ -1,14 +1,7 @@
-// G1 kernels of the plain (per-row) path: G1 FFT/IFFT of size 128/256/512, the fused
-// IFFT -> zero upper half -> FFT, the per-row MSM, setup gather, affine conversion and
-// element-wise G1/Fp arithmetic.
-//
-// The plain G1 transforms keep one transform per workgroup of n/2 threads with the n
-// points in LDS (packed Montgomery form, 144 B per point, 72 KiB for n = 512), one
-// butterfly per thread and stage: the structure of g1p_fft.cu:25-170 (forward) and
-// g1p_fft.cu:179-330 (inverse), with the inverse written as a decimation-in-time over
-// w^-1 followed by the 1/n scaling.  Twiddle multiplication uses the GLV halves of the root
-// (precomputed per root) with 2-bit joint windows -- 128 doublings and 64 additions instead
-// of the reference's 256-bit double-and-add (g1p_mul.cu:18-66).
+// G1 kernels of the drop-in entry points besides the transforms (which run on the
+// column-batched stage kernels of fast_kernels.hip): the MSM over per-call bases, the
+// h_fft upper-half fill, setup gather, affine conversion, compression and element-wise
+// G1/Fp arithmetic.
 #include "common.hip.h"
 
 namespace fk20 {
@@ -57,11 +50,6 @@ __device__ __forceinline__ void g1_gstore_std(uint32_t *g, const g1p &p) {
         reinterpret_cast<uint4 *>(g)[i] = make_uint4(w[4 * i], w[4 * i + 1], w[4 * i + 2], w[4 * i + 3]);
 }
 
-// Twiddle and 1/n multiplications: GLV halves with 2-bit joint windows (one shared copy).
-__device__ __noinline__ void g1_mul_glv_dev(g1p &r, const g1p &p, const uint32_t *kk) {
-    g1_mul_glv(r, p, kk);
-}
-
 // Scalar multiplication with a non-unrolled bit loop (one inlined dbl and add); any 256-bit k.
 __device__ __noinline__ void g1_mul_dev(g1p &r, const g1p &p, const uint32_t *k) {
     g1p q;
@@ -76,91 +64,6 @@ __device__ __noinline__ void g1_mul_dev(g1p &r, const g1p &p, const uint32_t *k)
         }
     }
     r = q;
-}
-
-// One radix-2 DIT butterfly on LDS points l and r with twiddle w^idx (w^-idx if INV).
-template <bool INV>
-__device__ __forceinline__ void g1_butterfly(uint32_t *lds, unsigned l, unsigned r, unsigned idx,
-                                             const DevTables *T) {
-    g1p x, y, a;
-    g1_lds_load(x, lds + r * 36);
-    if (idx != 0) g1_mul_glv_dev(x, x, T->tw_glv[INV ? 512 - idx : idx]);
-    g1_lds_load(y, lds + l * 36);
-    g1_add(a, y, x);
-    g1_neg(x, x);
-    g1_add(x, y, x);
-    g1_lds_store(lds + l * 36, a);
-    g1_lds_store(lds + r * 36, x);
-}
-
-// In-place DIT over n = 2^logn packed points in bit-reversed order -> natural order.
-template <bool INV>
-__device__ void g1_dit(uint32_t *lds, unsigned logn, const DevTables *T) {
-    const unsigned t = threadIdx.x;
-#pragma unroll 1
-    for (unsigned s = 0; s < logn; s++) {
-        const unsigned j = t & ((1u << s) - 1);
-        const unsigned l = ((t >> s) << (s + 1)) + j;
-        g1_butterfly<INV>(lds, l, l + (1u << s), j << (8 - s), T);
-        __syncthreads();
-    }
-}
-
-template <unsigned LOGN, bool INV>
-__global__ void __launch_bounds__(256) k_g1_fft(uint32_t *out, const uint32_t *in, const DevTables *T) {
-    constexpr unsigned n = 1u << LOGN, half = n / 2;
-    __shared__ __attribute__((aligned(16))) uint32_t lds[n * 36];
-    const unsigned t = threadIdx.x;
-    in += (size_t)blockIdx.x * n * G1P_WORDS;
-    out += (size_t)blockIdx.x * n * G1P_WORDS;
-    for (unsigned e = t; e < n; e += half) {
-        g1p p;
-        g1_gload_std(p, in + (size_t)e * G1P_WORDS);
-        g1_lds_store(lds + bitrev(e, LOGN) * 36, p);
-    }
-    __syncthreads();
-    g1_dit<INV>(lds, LOGN, T);
-    for (unsigned e = t; e < n; e += half) {
-        g1p p;
-        g1_lds_load(p, lds + e * 36);
-        if (INV) g1_mul_glv_dev(p, p, T->inv2k_glv[LOGN]);
-        g1_gstore_std(out + (size_t)e * G1P_WORDS, p);
-    }
-}
-
-// hext_fft -> IFFT -> keep the first half (scaled by 1/n), infinity above -> FFT.
-// fk20_hext_fft2h_fft.cu:27-70 (and FK20Py toeplitz_part3 + fk20_multi.py:98-104).
-template <unsigned LOGN>
-__global__ void __launch_bounds__(256) k_hext_fft2h_fft(uint32_t *h_fft, const uint32_t *hext_fft,
-                                                         const DevTables *T) {
-    constexpr unsigned n = 1u << LOGN, half = n / 2;
-    __shared__ __attribute__((aligned(16))) uint32_t lds[n * 36];
-    const unsigned t = threadIdx.x;
-    hext_fft += (size_t)blockIdx.x * n * G1P_WORDS;
-    h_fft += (size_t)blockIdx.x * n * G1P_WORDS;
-    for (unsigned e = t; e < n; e += half) {
-        g1p p;
-        g1_gload_std(p, hext_fft + (size_t)e * G1P_WORDS);
-        g1_lds_store(lds + bitrev(e, LOGN) * 36, p);
-    }
-    __syncthreads();
-    g1_dit<true>(lds, LOGN, T);
-    // natural order now; element t (< n/2) survives, scaled by 1/n; move it to the
-    // bit-reversed slot for the forward DIT and put infinity in the odd slots
-    g1p p;
-    g1_lds_load(p, lds + t * 36);
-    g1_mul_glv_dev(p, p, T->inv2k_glv[LOGN]);
-    __syncthreads();
-    const unsigned dst = bitrev(t, LOGN);
-    g1_lds_store(lds + dst * 36, p);
-    g1_inf(p);
-    g1_lds_store(lds + (dst | 1) * 36, p);
-    __syncthreads();
-    g1_dit<false>(lds, LOGN, T);
-    for (unsigned e = t; e < n; e += half) {
-        g1_lds_load(p, lds + e * 36);
-        g1_gstore_std(h_fft + (size_t)e * G1P_WORDS, p);
-    }
 }
 
 // h[row][n/2..n) = (0, 1, 0)  (fk20_poly2h_fft.cu:27)
@@ -201,8 +104,14 @@ __device__ __forceinline__ void msm_cols_add(g1p &acc, const uint32_t *tab, int 
     g1_add(acc, acc, X);
 }
 
-__global__ void __launch_bounds__(64, 2) k_msm_cols(uint32_t *hext, const uint32_t *tc_fft, const uint32_t *xpk,
-                                                 unsigned n2, unsigned rows) {
+// Scalar (row, i, j) at element row * row_stride + i * i_stride + j * j_stride; to_D: the
+// output is D[bitrev(j)][row] in packed Montgomery form, else hext_fft[row][j] standard form.
+struct MsmLayout {
+    unsigned row_stride, i_stride, j_stride, to_D;
+};
+
+__global__ void __launch_bounds__(64, 2) k_msm_cols(uint32_t *out, const uint32_t *tc_fft, const uint32_t *xpk,
+                                                    unsigned n2, unsigned rows, MsmLayout m) {
     __shared__ __attribute__((aligned(16))) uint32_t lds[16 * 8 * 36];
     __shared__ uint32_t kbuf[16 * 64];  // the current scalar word of each (base, lane)
     const unsigned j = blockIdx.x, t = threadIdx.x;
@@ -223,8 +132,8 @@ __global__ void __launch_bounds__(64, 2) k_msm_cols(uint32_t *hext, const uint32
     }
     __syncthreads();
     const unsigned row = min(blockIdx.y * 64 + t, rows - 1);  // lanes past the last row repeat it
-    const uint32_t *tc = tc_fft + ((size_t)row * 16 * n2 + j) * FR_WORDS;  // scalar i at tc[i n2 8]
-    const size_t istride = (size_t)n2 * FR_WORDS;
+    const uint32_t *tc = tc_fft + ((size_t)row * m.row_stride + (size_t)j * m.j_stride) * FR_WORDS;
+    const size_t istride = (size_t)m.i_stride * FR_WORDS;
     g1p acc;
     g1_inf(acc);
 #pragma unroll 1
@@ -253,7 +162,10 @@ __global__ void __launch_bounds__(64, 2) k_msm_cols(uint32_t *hext, const uint32
             }
         }
     }
-    if (blockIdx.y * 64 + t < rows) g1_gstore_std(hext + ((size_t)row * n2 + j) * G1P_WORDS, acc);
+    if (blockIdx.y * 64 + t < rows) {
+        if (m.to_D) g1_lds_store(out + ((size_t)bitrev(j, 31 - __builtin_clz(n2)) * rows + row) * 36, acc);
+        else g1_gstore_std(out + ((size_t)row * n2 + j) * G1P_WORDS, acc);
+    }
 }
 
 // xext[i][m] = srs[N-17-i-16m] for m <= k-2, infinity otherwise (FK20Py fk20_multi.py:82-85:
@@ -425,36 +337,6 @@ __global__ void k_g1_op(int op, uint32_t *out, const uint32_t *pp, const uint32_
 
 // ---------------------------------------------------------------- launchers
 
-template <bool INV>
-static hipError_t g1_fft_dispatch(uint32_t *out, const uint32_t *in, unsigned count, unsigned n,
-                                  const DevTables *T, hipStream_t s) {
-    switch (n) {
-    case 128: hipLaunchKernelGGL((k_g1_fft<7, INV>), dim3(count), dim3(64), 0, s, out, in, T); break;
-    case 256: hipLaunchKernelGGL((k_g1_fft<8, INV>), dim3(count), dim3(128), 0, s, out, in, T); break;
-    case 512: hipLaunchKernelGGL((k_g1_fft<9, INV>), dim3(count), dim3(256), 0, s, out, in, T); break;
-    default: return hipErrorInvalidValue;
-    }
-    return hipGetLastError();
-}
-
-hipError_t launch_g1_fft(uint32_t *out, const uint32_t *in, unsigned count, unsigned n, bool inverse,
-                         const DevTables *T, hipStream_t s) {
-    if (!count) return hipSuccess;
-    return inverse ? g1_fft_dispatch<true>(out, in, count, n, T, s) : g1_fft_dispatch<false>(out, in, count, n, T, s);
-}
-
-hipError_t launch_hext_fft2h_fft(uint32_t *h_fft, const uint32_t *hext_fft, unsigned rows, unsigned n2,
-                                 const DevTables *T, hipStream_t s) {
-    if (!rows) return hipSuccess;
-    switch (n2) {
-    case 128: hipLaunchKernelGGL(k_hext_fft2h_fft<7>, dim3(rows), dim3(64), 0, s, h_fft, hext_fft, T); break;
-    case 256: hipLaunchKernelGGL(k_hext_fft2h_fft<8>, dim3(rows), dim3(128), 0, s, h_fft, hext_fft, T); break;
-    case 512: hipLaunchKernelGGL(k_hext_fft2h_fft<9>, dim3(rows), dim3(256), 0, s, h_fft, hext_fft, T); break;
-    default: return hipErrorInvalidValue;
-    }
-    return hipGetLastError();
-}
-
 hipError_t launch_hext2h(uint32_t *h, unsigned rows, unsigned n2, hipStream_t s) {
     const size_t total = (size_t)rows * (n2 / 2);
     if (!total) return hipSuccess;
@@ -471,7 +353,18 @@ hipError_t launch_std_to_packed(uint32_t *packed, const uint32_t *pts, size_t co
 hipError_t launch_msm_plain(uint32_t *hext, const uint32_t *tc_fft, const uint32_t *xpk, unsigned rows,
                             unsigned n2, hipStream_t s) {
     if (!rows) return hipSuccess;
-    hipLaunchKernelGGL(k_msm_cols, dim3(n2, (rows + 63) / 64), dim3(64), 0, s, hext, tc_fft, xpk, n2, rows);
+    // scalars tc_fft[row][i][j], output hext_fft[row][j] (standard form)
+    const MsmLayout m = {16u * n2, n2, 1u, 0u};
+    hipLaunchKernelGGL(k_msm_cols, dim3(n2, (rows + 63) / 64), dim3(64), 0, s, hext, tc_fft, xpk, n2, rows, m);
+    return hipGetLastError();
+}
+
+hipError_t launch_msm_plain_T(uint32_t *D, const uint32_t *tcT, const uint32_t *xpk, unsigned rows, unsigned n2,
+                              hipStream_t s) {
+    if (!rows) return hipSuccess;
+    // scalars tcT[j][i][row], output D[bitrev(j)][row] (packed Montgomery)
+    const MsmLayout m = {1u, rows, 16u * rows, 1u};
+    hipLaunchKernelGGL(k_msm_cols, dim3(n2, (rows + 63) / 64), dim3(64), 0, s, D, tcT, xpk, n2, rows, m);
     return hipGetLastError();
 }
 

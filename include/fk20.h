@@ -70,7 +70,10 @@ int fr_fft(fk20_fr_t *output, const fk20_fr_t *input, unsigned count, unsigned n
 int fr_ift(fk20_fr_t *output, const fk20_fr_t *input, unsigned count, unsigned n,
            fk20_stream_t stream);
 
-/* count G1 FFTs of size n.  Replaces g1p_fft_wrapper (g1.cuh:96; g1p_fft.cu:337). */
+/* count G1 FFTs of size n.  Replaces g1p_fft_wrapper (g1.cuh:96; g1p_fft.cu:337).  The G1
+ * transforms (and fk20_hext_fft2h_fft, fk20_poly2h_fft, fk20_setup2xext_fft) stage their
+ * points in library-owned scratch of count * n * 144 bytes (grown on demand, per device,
+ * shared by these calls: not reentrant), so output may alias input. */
 int g1p_fft(fk20_g1p_t *output, const fk20_g1p_t *input, unsigned count, unsigned n,
             fk20_stream_t stream);
 /* Inverse G1 FFT including 1/n.  Replaces g1p_ift_wrapper (g1.cuh:97; g1p_fft.cu:350). */

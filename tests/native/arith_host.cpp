@@ -133,14 +133,6 @@ void host_g1_mul(uint64_t *r, const uint64_t *p, const uint64_t *k) {
     stp(r, a);
 }
 
-// r = (k1 + k2 lambda) p with kk = k1 (4 words) | k2 (4 words).
-void host_g1_mul_glv(uint64_t *r, const uint64_t *p, const uint32_t *kk) {
-    g1p a;
-    ldp(a, p);
-    g1_mul_glv(a, a, kk);
-    stp(r, a);
-}
-
 // Chain of mixed operations that keeps intermediate values in the lazy (unreduced) range.
 void host_g1_chain(uint64_t *r, const uint64_t *p, int steps) {
     g1p a, b;

@@ -157,23 +157,3 @@ def test_g1_compress_against_oracle(L):
         out = np.zeros(48, dtype=np.uint8)
         L.host_g1_compress(out.ctypes.data_as(ctypes.c_void_p), p64(p))
         assert bytes(out) == bytes(O.g1_compress(p))
-
-
-def test_g1_mul_glv_against_oracle(L):
-    """g1_mul_glv (GLV halves, 2-bit joint windows, complete additions) equals k P for
-    k = k2 lambda + k1: random scalars, the FFT roots, small and extreme halves, infinity."""
-    LAM = 0xAC45A4010001A40200000000FFFFFFFF
-    G = O.g1p_gen()
-    P2 = O.g1p_op("mul", G, O.fr(987654321))
-    w = pow(5, (R - 1) // 512, R)
-    ks = [0, 1, 2, LAM - 1, LAM, R - 1, pow(w, 1, R), pow(w, 511, R), pow(w, 256, R)]
-    ks += [int.from_bytes(np.random.default_rng(21 + i).bytes(32), "little") % R for i in range(8)]
-    for k in ks:
-        k2, k1 = divmod(k, LAM)
-        assert k1 < 2**128 and k2 < 2**128
-        kk = np.array([(k1 >> (32 * i)) & 0xFFFFFFFF for i in range(4)] +
-                      [(k2 >> (32 * i)) & 0xFFFFFFFF for i in range(4)], dtype=np.uint32)
-        for base in (G, P2, O.g1p_inf()):
-            r = np.zeros(18, dtype=np.uint64)
-            L.host_g1_mul_glv(p64(r), p64(np.ascontiguousarray(base)), kk.ctypes.data_as(ctypes.c_void_p))
-            assert O.g1p_eq(r, O.g1p_op("mul", base, O.fr(k))), hex(k)
