@@ -226,7 +226,8 @@ static int g1_transform(fk20_g1p_t *out, const fk20_g1p_t *in, unsigned count, u
     rc = device_scratch(st, (size_t)count * n * 144, &D);
     if (rc) return rc;
     return hip_status(launch_plain(inv ? PLAIN_G1_IFT : PLAIN_G1_FFT, W(out), W(in), nullptr, count, log2u(n),
-                                   reinterpret_cast<uint32_t *>(D), nullptr, st->tables, st->prog, S(stream)));
+                                   reinterpret_cast<uint32_t *>(D), nullptr, nullptr, st->tables, st->prog,
+                                   S(stream)));
 }
 
 int g1p_fft(fk20_g1p_t *output, const fk20_g1p_t *input, unsigned count, unsigned n, fk20_stream_t stream) {
@@ -237,14 +238,8 @@ int g1p_ift(fk20_g1p_t *output, const fk20_g1p_t *input, unsigned count, unsigne
     return g1_transform(output, input, count, n, true, stream);
 }
 
-// MSM with the bases converted once per call into packed Montgomery form in scratch.
-static int msm_with_scratch(fk20_g1p_t *hext_fft, const fk20_fr_t *tc_fft, const fk20_g1p_t *xext_fft,
-                            unsigned rows, unsigned n2, void *xpk, hipStream_t s) {
-    hipError_t e = launch_std_to_packed(W(xpk), W(xext_fft), (size_t)16 * n2, s);
-    if (e != hipSuccess) return hip_status(e);
-    return hip_status(launch_msm_plain(W(hext_fft), W(tc_fft), W(xpk), rows, n2, s));
-}
-
+// MSM with the bases converted once per call into packed Montgomery form in scratch,
+// followed there by the scalars' Booth digits.
 int fk20_msm(fk20_g1p_t *hext_fft, const fk20_fr_t *tc_fft, const fk20_g1p_t *xext_fft, unsigned rows,
              unsigned n2, fk20_stream_t stream) {
     if (!valid_n(n2) || (rows && (!hext_fft || !tc_fft || !xext_fft))) return FK20_EINVAL;
@@ -252,10 +247,15 @@ int fk20_msm(fk20_g1p_t *hext_fft, const fk20_fr_t *tc_fft, const fk20_g1p_t *xe
     DeviceState *st;
     int rc = device_state(&st);
     if (rc) return rc;
+    const size_t x_bytes = (size_t)16 * n2 * 144;
     void *scratch;
-    rc = device_scratch(st, (size_t)16 * n2 * 144, &scratch);
+    rc = device_scratch(st, x_bytes + msm_plain_digit_bytes(rows, n2), &scratch);
     if (rc) return rc;
-    return msm_with_scratch(hext_fft, tc_fft, xext_fft, rows, n2, scratch, S(stream));
+    uint32_t *xpk = reinterpret_cast<uint32_t *>(scratch);
+    int8_t *digits = reinterpret_cast<int8_t *>(scratch) + x_bytes;
+    hipStream_t s = S(stream);
+    FK20_TRY(launch_std_to_packed(xpk, W(xext_fft), (size_t)16 * n2, s));
+    return hip_status(launch_msm_plain(W(hext_fft), W(tc_fft), xpk, digits, rows, n2, s));
 }
 
 int fk20_hext2h(fk20_g1p_t *h, unsigned rows, unsigned n2, fk20_stream_t stream) {
@@ -274,7 +274,8 @@ int fk20_hext_fft2h_fft(fk20_g1p_t *h_fft, const fk20_g1p_t *hext_fft, unsigned 
     rc = device_scratch(st, (size_t)rows * n2 * 144, &D);
     if (rc) return rc;
     return hip_status(launch_plain(PLAIN_HEXT_FFT2H_FFT, W(h_fft), W(hext_fft), nullptr, rows, log2u(n2),
-                                   reinterpret_cast<uint32_t *>(D), nullptr, st->tables, st->prog, S(stream)));
+                                   reinterpret_cast<uint32_t *>(D), nullptr, nullptr, st->tables, st->prog,
+                                   S(stream)));
 }
 
 // fk20_poly2h_fft.cu:47-80: toeplitz -> NTT (16 per row) -> MSM -> IFFT -> zero -> FFT, with
@@ -292,16 +293,17 @@ int fk20_poly2h_fft(fk20_g1p_t *h_fft, const fk20_fr_t *poly, const fk20_g1p_t *
     const size_t d_bytes = (size_t)rows * n2 * 144;
     const size_t x_bytes = (size_t)16 * n2 * 144;
     void *scratch;
-    rc = device_scratch(st, tc_bytes + d_bytes + x_bytes, &scratch);
+    rc = device_scratch(st, tc_bytes + d_bytes + x_bytes + msm_plain_digit_bytes(rows, n2), &scratch);
     if (rc) return rc;
     char *base = reinterpret_cast<char *>(scratch);
     uint32_t *tcT = reinterpret_cast<uint32_t *>(base);
     uint32_t *D = reinterpret_cast<uint32_t *>(base + tc_bytes);
     uint32_t *xpk = reinterpret_cast<uint32_t *>(base + tc_bytes + d_bytes);
+    int8_t *digits = reinterpret_cast<int8_t *>(base + tc_bytes + d_bytes + x_bytes);
     hipStream_t s = S(stream);
     FK20_TRY(launch_std_to_packed(xpk, W(xext_fft), (size_t)16 * n2, s));
-    return hip_status(launch_plain(PLAIN_POLY2H_FFT, W(h_fft), W(poly), xpk, rows, log2u(n2), D, tcT, st->tables,
-                                   st->prog, s));
+    return hip_status(launch_plain(PLAIN_POLY2H_FFT, W(h_fft), W(poly), xpk, rows, log2u(n2), D, tcT, digits,
+                                   st->tables, st->prog, s));
 }
 
 int fk20_setup2xext_fft(fk20_g1p_t *xext_fft, const fk20_g1p_t *setup, unsigned N, fk20_stream_t stream) {
@@ -315,7 +317,7 @@ int fk20_setup2xext_fft(fk20_g1p_t *xext_fft, const fk20_g1p_t *setup, unsigned 
     hipStream_t s = S(stream);
     FK20_TRY(launch_setup_gather(W(xext_fft), W(setup), N, s));
     return hip_status(launch_plain(PLAIN_G1_FFT, W(xext_fft), W(xext_fft), nullptr, 16, log2u(N / 8),
-                                   reinterpret_cast<uint32_t *>(D), nullptr, st->tables, st->prog, s));
+                                   reinterpret_cast<uint32_t *>(D), nullptr, nullptr, st->tables, st->prog, s));
 }
 
 // ---------------------------------------------------------------- helpers

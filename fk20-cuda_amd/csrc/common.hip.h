@@ -49,12 +49,15 @@ hipError_t launch_fr_ntt(uint32_t *out, const uint32_t *in, unsigned count, unsi
                          const DevTables *T, hipStream_t s);
 hipError_t launch_hext2h(uint32_t *h, unsigned rows, unsigned n2, hipStream_t s);
 hipError_t launch_std_to_packed(uint32_t *packed, const uint32_t *std_pts, size_t count, hipStream_t s);
-hipError_t launch_msm_plain(uint32_t *hext_fft, const uint32_t *tc_fft, const uint32_t *xext_packed,
+// fk20_msm over per-call bases (16 x n2 packed Montgomery points); `digits` is workspace of
+// msm_plain_digit_bytes(rows, n2) bytes (the scalars' Booth digits).
+size_t msm_plain_digit_bytes(unsigned rows, unsigned n2);
+hipError_t launch_msm_plain(uint32_t *hext_fft, const uint32_t *tc_fft, const uint32_t *xext_packed, int8_t *digits,
                             unsigned rows, unsigned n2, hipStream_t s);
 // The same MSM from transposed scalars tcT[j][i][row] into D[bitrev(j)][row] (packed
 // Montgomery), the layouts of the column-batched transforms.
-hipError_t launch_msm_plain_T(uint32_t *D, const uint32_t *tcT, const uint32_t *xext_packed, unsigned rows,
-                              unsigned n2, hipStream_t s);
+hipError_t launch_msm_plain_T(uint32_t *D, const uint32_t *tcT, const uint32_t *xext_packed, int8_t *digits,
+                              unsigned rows, unsigned n2, hipStream_t s);
 hipError_t launch_setup_gather(uint32_t *xext, const uint32_t *srs, unsigned N, hipStream_t s);
 hipError_t launch_to_affine(uint32_t *out, const uint32_t *in, size_t count, hipStream_t s);
 hipError_t launch_compress(uint32_t *out, const uint32_t *in, size_t count, hipStream_t s);

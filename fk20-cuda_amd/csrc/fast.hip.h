@@ -105,7 +105,8 @@ hipError_t launch_batch(const Launch &L, const DevTables *T, const uint32_t *ops
 
 // The drop-in entry points on the same column-batched transform kernels, for `rows`
 // transforms of n = 2^logn points (n2 = n for the FK20 kinds), with D = rows * n * 144
-// bytes of workspace (and tcT = rows * 16 * n * 32 for PLAIN_POLY2H_FFT):
+// bytes of workspace (and, for PLAIN_POLY2H_FFT, tcT = rows * 16 * n * 32 bytes and
+// digits = msm_plain_digit_bytes(rows, n)):
 //   PLAIN_G1_FFT          out[r] = DFT(in[r])                         (g1p_fft)
 //   PLAIN_G1_IFT          out[r] = DFT^-1(in[r]), scaled by 1/n       (g1p_ift)
 //   PLAIN_HEXT_FFT2H_FFT  IFFT, zero upper half, FFT                  (fk20_hext_fft2h_fft)
@@ -114,7 +115,7 @@ hipError_t launch_batch(const Launch &L, const DevTables *T, const uint32_t *ops
 // out may alias in: every input is read into D before the first output is written.
 enum PlainKind { PLAIN_G1_FFT = 0, PLAIN_G1_IFT = 1, PLAIN_HEXT_FFT2H_FFT = 2, PLAIN_POLY2H_FFT = 3 };
 hipError_t launch_plain(PlainKind kind, uint32_t *out, const uint32_t *in, const uint32_t *xpk, unsigned rows,
-                        unsigned logn, uint32_t *D, uint32_t *tcT, const DevTables *T, const Programs &P,
-                        hipStream_t s);
+                        unsigned logn, uint32_t *D, uint32_t *tcT, int8_t *digits, const DevTables *T,
+                        const Programs &P, hipStream_t s);
 
 }  // namespace fk20

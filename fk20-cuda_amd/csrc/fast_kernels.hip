@@ -765,8 +765,8 @@ hipError_t launch_batch(const Launch &base, const DevTables *T, const uint32_t *
 // stages 0 .. L-1 (the first likewise), and the output kernel writes natural order from the
 // DIF's bit-reversed positions (format 0) or position by position (format 1).
 hipError_t launch_plain(PlainKind kind, uint32_t *out, const uint32_t *in, const uint32_t *xpk, unsigned rows,
-                        unsigned logn, uint32_t *D, uint32_t *tcT, const DevTables *T, const Programs &P,
-                        hipStream_t s) {
+                        unsigned logn, uint32_t *D, uint32_t *tcT, int8_t *digits, const DevTables *T,
+                        const Programs &P, hipStream_t s) {
     if (!rows) return hipSuccess;
     Launch L;
     memset(&L, 0, sizeof L);
@@ -807,7 +807,7 @@ hipError_t launch_plain(PlainKind kind, uint32_t *out, const uint32_t *in, const
         } else {
             per[0] = rows * 16;
             hipLaunchKernelGGL(k_toeplitz_fft_multi, dim3(set_blocks(L, per)), dim3(256), 0, s, L, T);
-            hipError_t e = launch_msm_plain_T(D, tcT, xpk, rows, n, s);
+            hipError_t e = launch_msm_plain_T(D, tcT, xpk, digits, rows, n, s);
             if (e != hipSuccess) return e;
         }
         for (unsigned t = 0; t + 1 < logn; t++) stage(k_dit_multi<TW_WNAF>, (int)t);

@@ -2,6 +2,8 @@
 // column-batched stage kernels of fast_kernels.hip): the MSM over per-call bases, the
 // h_fft upper-half fill, setup gather, affine conversion, compression and element-wise
 // G1/Fp arithmetic.
+#include <algorithm>
+
 #include "common.hip.h"
 
 namespace fk20 {
@@ -88,14 +90,52 @@ __global__ void k_std_to_packed(uint32_t *packed, const uint32_t *pts, size_t co
 }
 
 // hext_fft[row][j] = sum_i tc_fft[row][i][j] * xext[i][j] (fk20_msm.cu:23-66), for any 256-bit
-// scalars.  A workgroup is one column j for 64 rows (one row per lane), so the 16 bases are
-// shared: their multiples 1..8 X_i are built once into LDS (16 x 8 points, 18 KiB) and every
-// lane runs Straus over radix-16 Booth digits of its 16 scalars,
-//     d_w = b[4w-1] + b[4w] + 2 b[4w+1] + 4 b[4w+2] - 8 b[4w+3]  in [-8, 8],  b[-1] = 0,
-// so that sum_{w<64} d_w 16^w = k - b[255] 2^256: the top bit is one more window d_64 = b[255].
-// Every digit reads 5 scalar bits and no carries, so all lanes run the same steps: 64
-// windows of 4 doublings and 16 complete additions (digit 0 adds infinity).
-__device__ __forceinline__ void msm_cols_add(g1p &acc, const uint32_t *tab, int dg) {
+// scalars, in two kernels.
+//
+// k_msm_digits rewrites every scalar as 43 signed radix-64 Booth digits
+//     d_w = b[6w-1] + b[6w] + 2 b[6w+1] + 4 b[6w+2] + 8 b[6w+3] + 16 b[6w+4] - 32 b[6w+5]
+// (b[-1] = 0, b[256..] = 0), d_w in [-32, 32] and sum_w d_w 64^w = k: each digit reads 7 bits
+// and no carries.  Digits go to digits[j][i][w][row] (int8), so the MSM's 64 lanes (rows)
+// read 64 consecutive bytes per (term, window).
+//
+// k_msm_booth: a workgroup is one column j for up to 256 rows (one row per lane), so the
+// column's 16 bases are shared: their multiples 1..32 X_i are built once into LDS (16 x 32
+// points, 72 KiB) and every lane runs Straus over its 16 digit strings -- 42 windows of 6
+// doublings and 43 x 16 complete additions in all, the same steps in every lane (digit 0
+// adds infinity).
+static constexpr unsigned MSM_NWIN = 43, MSM_HALF = 32;
+
+// Scalar (row, i, j) at element row * row_stride + i * i_stride + j * j_stride; to_D: the
+// output is D[bitrev(j)][row] in packed Montgomery form, else hext_fft[row][j] standard form.
+struct MsmLayout {
+    unsigned row_stride, i_stride, j_stride, to_D;
+};
+
+__global__ void __launch_bounds__(256) k_msm_digits(int8_t *digits, const uint32_t *tc, unsigned n2, unsigned rows,
+                                                    MsmLayout m) {
+    const size_t e = (size_t)blockIdx.x * 256 + threadIdx.x;  // (j, i, row), row fastest
+    if (e >= (size_t)n2 * 16 * rows) return;
+    const unsigned row = (unsigned)(e % rows), i = (unsigned)((e / rows) % 16), j = (unsigned)(e / rows / 16);
+    uint32_t k[9];
+    words_load(k, tc + ((size_t)row * m.row_stride + (size_t)i * m.i_stride + (size_t)j * m.j_stride) * FR_WORDS, 8);
+    // K = 2k (257 bits, k[8] = the carry out), so field w is bits 6w .. 6w+6 of K
+    k[8] = k[7] >> 31;
+#pragma unroll
+    for (int q = 7; q > 0; q--) k[q] = (k[q] << 1) | (k[q - 1] >> 31);
+    k[0] <<= 1;
+    int8_t *out = digits + ((size_t)j * 16 + i) * MSM_NWIN * rows + row;
+#pragma unroll
+    for (unsigned w = 0; w < MSM_NWIN; w++) {
+        const unsigned p = 6 * w, q = p / 32, sh = p % 32;
+        uint32_t f = k[q] >> sh;
+        if (sh > 25 && q < 8) f |= k[q + 1] << (32 - sh);
+        f &= 127u;
+        const int d = (int)(f & 1u) + (int)((f >> 1) & 31u) - (int)((f >> 6) & 1u) * 32;
+        out[(size_t)w * rows] = (int8_t)d;
+    }
+}
+
+__device__ __forceinline__ void msm_booth_add(g1p &acc, const uint32_t *tab, int dg) {
     const unsigned a = (unsigned)(dg < 0 ? -dg : dg);
     g1p X;
     g1_lds_load(X, tab + (a ? a - 1 : 0) * 36);
@@ -104,66 +144,41 @@ __device__ __forceinline__ void msm_cols_add(g1p &acc, const uint32_t *tab, int 
     g1_add(acc, acc, X);
 }
 
-// Scalar (row, i, j) at element row * row_stride + i * i_stride + j * j_stride; to_D: the
-// output is D[bitrev(j)][row] in packed Montgomery form, else hext_fft[row][j] standard form.
-struct MsmLayout {
-    unsigned row_stride, i_stride, j_stride, to_D;
-};
-
-__global__ void __launch_bounds__(64, 2) k_msm_cols(uint32_t *out, const uint32_t *tc_fft, const uint32_t *xpk,
-                                                    unsigned n2, unsigned rows, MsmLayout m) {
-    __shared__ __attribute__((aligned(16))) uint32_t lds[16 * 8 * 36];
-    __shared__ uint32_t kbuf[16 * 64];  // the current scalar word of each (base, lane)
+__global__ void __launch_bounds__(256, 2) k_msm_booth(uint32_t *out, const int8_t *digits, const uint32_t *xpk,
+                                                     unsigned n2, unsigned rows, unsigned to_D) {
+    __shared__ __attribute__((aligned(16))) uint32_t lds[16 * MSM_HALF * 36];
     const unsigned j = blockIdx.x, t = threadIdx.x;
-    {
-        // lane t builds d X_i and (d + 1) X_i, i = t / 4, d = 2 (t % 4) + 1; entry a - 1 holds a X_i
-        const unsigned i = t / 4, d = 2 * (t % 4) + 1;
-        g1p X, e;
+    // entry e = 32 i + (m - 1) holds m X_i, by double-and-add over m's 6 bits
+    for (unsigned e = t; e < 16 * MSM_HALF; e += blockDim.x) {
+        const unsigned i = e / MSM_HALF, mult = e % MSM_HALF + 1;
+        g1p X, acc;
         g1_lds_load(X, xpk + ((size_t)i * n2 + j) * 36);  // packed Montgomery points in HBM
-        g1_inf(e);
+        g1_inf(acc);
 #pragma unroll 1
-        for (int b = 2; b >= 0; b--) {
-            g1_dbl(e, e);
-            if ((d >> b) & 1) g1_add(e, e, X);
+        for (int b = 5; b >= 0; b--) {
+            g1_dbl(acc, acc);
+            if ((mult >> b) & 1) g1_add(acc, acc, X);
         }
-        g1_lds_store(lds + (i * 8 + d - 1) * 36, e);
-        g1_add(e, e, X);
-        g1_lds_store(lds + (i * 8 + d) * 36, e);
+        g1_lds_store(lds + e * 36, acc);
     }
     __syncthreads();
-    const unsigned row = min(blockIdx.y * 64 + t, rows - 1);  // lanes past the last row repeat it
-    const uint32_t *tc = tc_fft + ((size_t)row * m.row_stride + (size_t)j * m.j_stride) * FR_WORDS;
-    const size_t istride = (size_t)m.i_stride * FR_WORDS;
+    const unsigned row0 = blockIdx.y * blockDim.x + t;
+    const unsigned row = min(row0, rows - 1);  // lanes past the last row repeat it
+    const int8_t *dg = digits + (size_t)j * 16 * MSM_NWIN * rows + row;
     g1p acc;
     g1_inf(acc);
 #pragma unroll 1
-    for (unsigned i = 0; i < 16; i++) msm_cols_add(acc, lds + i * 8 * 36, (int)(tc[i * istride + 7] >> 31));
+    for (int w = MSM_NWIN - 1; w >= 0; w--) {
+        if (w != (int)MSM_NWIN - 1) {
 #pragma unroll 1
-    for (int wd = 7; wd >= 0; wd--) {
-        uint32_t low = 0;  // bit i: bit 32 wd - 1 of scalar i
-#pragma unroll
-        for (int i = 0; i < 16; i++) {
-            kbuf[i * 64 + t] = tc[i * istride + wd];  // each lane reads back only its own words
-            if (wd > 0) low |= (tc[i * istride + wd - 1] >> 31) << i;
+            for (int d = 0; d < 6; d++) g1_dbl(acc, acc);
         }
 #pragma unroll 1
-        for (int q = 7; q >= 0; q--) {
-            g1_dbl(acc, acc);
-            g1_dbl(acc, acc);
-            g1_dbl(acc, acc);
-            g1_dbl(acc, acc);
-#pragma unroll 1
-            for (unsigned i = 0; i < 16; i++) {
-                const uint32_t kw = kbuf[i * 64 + t];
-                const uint32_t bits = kw >> (4 * q);
-                const uint32_t below = q ? (kw >> (4 * q - 1)) & 1u : (low >> i) & 1u;
-                const int dg = (int)(bits & 7u) + (int)below - (int)(bits & 8u);
-                msm_cols_add(acc, lds + i * 8 * 36, dg);
-            }
-        }
+        for (unsigned i = 0; i < 16; i++)
+            msm_booth_add(acc, lds + i * MSM_HALF * 36, dg[((size_t)i * MSM_NWIN + w) * rows]);
     }
-    if (blockIdx.y * 64 + t < rows) {
-        if (m.to_D) g1_lds_store(out + ((size_t)bitrev(j, 31 - __builtin_clz(n2)) * rows + row) * 36, acc);
+    if (row0 < rows) {
+        if (to_D) g1_lds_store(out + ((size_t)bitrev(j, 31 - __builtin_clz(n2)) * rows + row) * 36, acc);
         else g1_gstore_std(out + ((size_t)row * n2 + j) * G1P_WORDS, acc);
     }
 }
@@ -350,22 +365,29 @@ hipError_t launch_std_to_packed(uint32_t *packed, const uint32_t *pts, size_t co
     return hipGetLastError();
 }
 
-hipError_t launch_msm_plain(uint32_t *hext, const uint32_t *tc_fft, const uint32_t *xpk, unsigned rows,
-                            unsigned n2, hipStream_t s) {
+size_t msm_plain_digit_bytes(unsigned rows, unsigned n2) { return (size_t)rows * n2 * 16 * MSM_NWIN; }
+
+static hipError_t msm_plain(uint32_t *out, const uint32_t *tc, const uint32_t *xpk, int8_t *digits, unsigned rows,
+                            unsigned n2, MsmLayout m, hipStream_t s) {
     if (!rows) return hipSuccess;
-    // scalars tc_fft[row][i][j], output hext_fft[row][j] (standard form)
-    const MsmLayout m = {16u * n2, n2, 1u, 0u};
-    hipLaunchKernelGGL(k_msm_cols, dim3(n2, (rows + 63) / 64), dim3(64), 0, s, hext, tc_fft, xpk, n2, rows, m);
+    const size_t scalars = (size_t)rows * n2 * 16;
+    hipLaunchKernelGGL(k_msm_digits, dim3((unsigned)((scalars + 255) / 256)), dim3(256), 0, s, digits, tc, n2, rows, m);
+    const unsigned threads = 64 * std::min(4u, (rows + 63) / 64);
+    hipLaunchKernelGGL(k_msm_booth, dim3(n2, (rows + threads - 1) / threads), dim3(threads), 0, s, out,
+                       (const int8_t *)digits, xpk, n2, rows, m.to_D);
     return hipGetLastError();
 }
 
-hipError_t launch_msm_plain_T(uint32_t *D, const uint32_t *tcT, const uint32_t *xpk, unsigned rows, unsigned n2,
-                              hipStream_t s) {
-    if (!rows) return hipSuccess;
+hipError_t launch_msm_plain(uint32_t *hext, const uint32_t *tc_fft, const uint32_t *xpk, int8_t *digits,
+                            unsigned rows, unsigned n2, hipStream_t s) {
+    // scalars tc_fft[row][i][j], output hext_fft[row][j] (standard form)
+    return msm_plain(hext, tc_fft, xpk, digits, rows, n2, MsmLayout{16u * n2, n2, 1u, 0u}, s);
+}
+
+hipError_t launch_msm_plain_T(uint32_t *D, const uint32_t *tcT, const uint32_t *xpk, int8_t *digits, unsigned rows,
+                              unsigned n2, hipStream_t s) {
     // scalars tcT[j][i][row], output D[bitrev(j)][row] (packed Montgomery)
-    const MsmLayout m = {1u, rows, 16u * rows, 1u};
-    hipLaunchKernelGGL(k_msm_cols, dim3(n2, (rows + 63) / 64), dim3(64), 0, s, D, tcT, xpk, n2, rows, m);
-    return hipGetLastError();
+    return msm_plain(D, tcT, xpk, digits, rows, n2, MsmLayout{1u, rows, 16u * rows, 1u}, s);
 }
 
 hipError_t launch_setup_gather(uint32_t *xext, const uint32_t *srs, unsigned N, hipStream_t s) {
