@@ -2,6 +2,7 @@
 // workspace and optional hipGraph replay.  See include/fk20.h for the interface.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cstring>
 #include <mutex>
 #include <vector>
@@ -53,17 +54,66 @@ std::vector<int> wnaf(u128 k, int W) {
     return d;
 }
 
-void append_program(std::vector<uint32_t> &ops, const uint32_t *k, int W) {
-    u128 k2, k1;
-    divmod_lambda(k, k2, k1);
-    const std::vector<int> d1 = wnaf(k1, W), d2 = wnaf(k2, W);
+// Ladder cost of a program over digit strings d1, d2 in Fp multiplications: one doubling (8)
+// per position below the top, one complete addition (12) per non-zero digit after the first.
+int program_cost(const std::vector<int> &d1, const std::vector<int> &d2) {
     const int L = (int)std::max(d1.size(), d2.size());
+    int nz = 0;
+    for (int x : d1) nz += x != 0;
+    for (int x : d2) nz += x != 0;
+    return std::max(L - 1, 0) * 8 + std::max(nz - 1, 0) * 12;
+}
+
+// k * P as a program over the wNAF digits of a split of k.  Four splits are tried and the
+// cheapest kept: k or r - k (then every digit negated, since (r - k) P = -k P), each as
+// k = k2 lambda + k1 with 0 <= k1 < lambda or with k1 - lambda (negative) and k2 + 1.  The
+// choice matters for some roots: w^384 = -w^128 where w^128 = z^3 = z lambda + z has 64-bit
+// halves.
+void append_program(std::vector<uint32_t> &ops, const uint32_t *k, int W) {
+    const u128 lam = ((u128)GLV_LAMBDA[1] << 64) | GLV_LAMBDA[0];
+    std::vector<int> b1, b2;
+    int best = -1;
+    for (int neg = 0; neg < 2; neg++) {
+        uint32_t kk[8];
+        if (neg) {  // r - k (k = 0 has no negation to try)
+            bool zero = true;
+            for (int i = 0; i < 8; i++) zero = zero && k[i] == 0;
+            if (zero) continue;
+            uint64_t borrow = 0;
+            for (int i = 0; i < 8; i++) {
+                const uint64_t d = (uint64_t)FR_R[i] - k[i] - borrow;
+                kk[i] = (uint32_t)d;
+                borrow = (d >> 63) & 1;
+            }
+        } else {
+            memcpy(kk, k, 32);
+        }
+        u128 k2, k1;
+        divmod_lambda(kk, k2, k1);
+        for (int shifted = 0; shifted < 2; shifted++) {
+            // shifted: k1 - lambda < 0 and k2 + 1
+            std::vector<int> d1 = wnaf(shifted ? lam - k1 : k1, W), d2 = wnaf(shifted ? k2 + 1 : k2, W);
+            if (shifted && k1 == 0) continue;
+            if (shifted) for (int &x : d1) x = -x;
+            if (neg) {
+                for (int &x : d1) x = -x;
+                for (int &x : d2) x = -x;
+            }
+            const int c = program_cost(d1, d2);
+            if (best < 0 || c < best) {
+                best = c;
+                b1 = d1;
+                b2 = d2;
+            }
+        }
+    }
+    const int L = (int)std::max(b1.size(), b2.size());
     bool started = false;
     uint32_t pending = 0;
     for (int t = L - 1; t >= 0; t--) {
         if (started) pending++;
         for (int part = 0; part < 2; part++) {
-            const std::vector<int> &d = part ? d2 : d1;
+            const std::vector<int> &d = part ? b2 : b1;
             const int digit = t < (int)d.size() ? d[t] : 0;
             if (!digit) continue;
             const uint32_t e = (uint32_t)((digit < 0 ? -digit : digit) - 1) / 2;

@@ -82,7 +82,8 @@ def launch_mul32(N, kind, rows):
 #
 # Fp multiplications the batched path's algorithms actually perform (same 300-mul32
 # convention), for the roofline of its kernels:
-#   twiddle w^e: GLV split w^e = k2*lambda + k1, wNAF-5 digits of k1 and k2; table
+#   twiddle w^e: GLV split w^e = k2*lambda + k1 (the cheapest of k or r - k, each with
+#     0 <= k1 < lambda or k1 - lambda, k2 + 1), wNAF-5 digits of k1 and k2; table
 #     P, 3P, .., 15P (1 dbl + 7 add); one dbl per digit position after the first, one add
 #     per non-zero digit after the first, one Fp mul per phi-digit
 #   butterfly: twiddle (if e != 0) + 2 complete additions
@@ -109,12 +110,42 @@ def _wnaf(k, w):
     return d
 
 
+def _program_cost(d1, d2):
+    L = max(len(d1), len(d2))
+    nz = sum(1 for x in d1 if x) + sum(1 for x in d2 if x)
+    return max(L - 1, 0) * 8 + max(nz - 1, 0) * 12
+
+
+def glv_digits(k):
+    """The digit strings (k1, k2) the library's program for k * P uses: of the splits of k
+    and of r - k (then negated), each with 0 <= k1 < lambda or k1 - lambda and k2 + 1, the
+    first of least ladder cost (8 per doubling, 12 per addition)."""
+    best = None
+    for neg in (False, True):
+        if neg and k == 0:
+            continue
+        kk = (R_MOD - k) if neg else k
+        k2, k1 = divmod(kk, GLV_LAMBDA)
+        for shifted in (False, True):
+            if shifted and k1 == 0:
+                continue
+            d1 = _wnaf(GLV_LAMBDA - k1 if shifted else k1, WNAF_W)
+            d2 = _wnaf(k2 + 1 if shifted else k2, WNAF_W)
+            if shifted:
+                d1 = [-x for x in d1]
+            if neg:
+                d1, d2 = [-x for x in d1], [-x for x in d2]
+            c = _program_cost(d1, d2)
+            if best is None or c < best[0]:
+                best = (c, d1, d2)
+    return best[1], best[2]
+
+
 def twiddle_program(e_idx, inverse=False):
     """(doublings, additions, phi_muls, ops) of the program for w^e (w of order 512)."""
     w = pow(5, (R_MOD - 1) // 512, R_MOD)
     k = pow(w, (512 - e_idx) % 512 if inverse else e_idx, R_MOD)
-    k2, k1 = divmod(k, GLV_LAMBDA)
-    d1, d2 = _wnaf(k1, WNAF_W), _wnaf(k2, WNAF_W)
+    d1, d2 = glv_digits(k)
     L = max(len(d1), len(d2))
     dbl = adds = phi = ops = 0
     started = False
