@@ -578,9 +578,9 @@ static hipError_t enqueue_items(fk20_ctx *c, const fk20_batch_item_t *items, uns
     // up front took queues that the server's two slot streams then had to share -- serving
     // throughput at saturation measured 8.8 K rows/s that way against 11.2 K created lazily.
     hipError_t e = hipSuccess;
-    for (int k = 0; k < 2 && e == hipSuccess && !c->sub[k]; k++) {
-        e = hipStreamCreateWithFlags(&c->sub[k], hipStreamNonBlocking);
-        if (e == hipSuccess) e = hipEventCreateWithFlags(&c->join[k], hipEventDisableTiming);
+    for (int k = 0; k < 2 && e == hipSuccess; k++) {
+        if (!c->sub[k]) e = hipStreamCreateWithFlags(&c->sub[k], hipStreamNonBlocking);
+        if (e == hipSuccess && !c->join[k]) e = hipEventCreateWithFlags(&c->join[k], hipEventDisableTiming);
     }
     if (e == hipSuccess && !c->fork) e = hipEventCreateWithFlags(&c->fork, hipEventDisableTiming);
     if (e == hipSuccess) e = hipEventRecord(c->fork, s);
