@@ -64,46 +64,72 @@ int program_cost(const std::vector<int> &d1, const std::vector<int> &d2) {
     return std::max(L - 1, 0) * 8 + std::max(nz - 1, 0) * 12;
 }
 
-// k * P as a program over the wNAF digits of a split of k.  Four splits are tried and the
-// cheapest kept: k or r - k (then every digit negated, since (r - k) P = -k P), each as
-// k = k2 lambda + k1 with 0 <= k1 < lambda or with k1 - lambda (negative) and k2 + 1.  The
-// choice matters for some roots: w^384 = -w^128 where w^128 = z^3 = z lambda + z has 64-bit
-// halves.
+// lambda (the eigenvalue of phi on G1) in Montgomery form
+fs lambda_mont() {
+    fs lam;
+    fs_zero(lam);
+    lam.l[0] = (uint32_t)GLV_LAMBDA[0];
+    lam.l[1] = (uint32_t)(GLV_LAMBDA[0] >> 32);
+    lam.l[2] = (uint32_t)GLV_LAMBDA[1];
+    lam.l[3] = (uint32_t)(GLV_LAMBDA[1] >> 32);
+    fs_to_mont(lam, lam);
+    return lam;
+}
+
+// k * P as a program over the wNAF digits of a split of k.  Twelve splits are tried and the
+// cheapest kept: k lambda^-t for t = 0, 1, 2 (then k1's digits act on phi^t(P) and k2's on
+// phi^(t+1)(P)), each as itself or as r minus it (then every digit negated, since
+// (r - k) P = -k P), each with 0 <= k1 < lambda or with k1 - lambda and k2 + 1.  The choice
+// matters most for w^-128 = -w^128 = -(z lambda + z), whose halves are 64-bit.
 void append_program(std::vector<uint32_t> &ops, const uint32_t *k, int W) {
     const u128 lam = ((u128)GLV_LAMBDA[1] << 64) | GLV_LAMBDA[0];
+    const fs lam_m = lambda_mont();
+    fs lam_inv;  // lambda^-1 = lambda^2
+    fs_mul(lam_inv, lam_m, lam_m);
+    fs km, kt;
+    fs_zero(km);
+    for (int i = 0; i < 8; i++) km.l[i] = k[i];
+    fs_to_mont(km, km);
     std::vector<int> b1, b2;
-    int best = -1;
-    for (int neg = 0; neg < 2; neg++) {
-        uint32_t kk[8];
-        if (neg) {  // r - k (k = 0 has no negation to try)
-            bool zero = true;
-            for (int i = 0; i < 8; i++) zero = zero && k[i] == 0;
-            if (zero) continue;
-            uint64_t borrow = 0;
-            for (int i = 0; i < 8; i++) {
-                const uint64_t d = (uint64_t)FR_R[i] - k[i] - borrow;
-                kk[i] = (uint32_t)d;
-                borrow = (d >> 63) & 1;
+    int best = -1, best_t = 0;
+    kt = km;
+    for (int t = 0; t < 3; t++) {
+        if (t) fs_mul(kt, kt, lam_inv);
+        fs ks;
+        fs_from_mont(ks, kt);
+        for (int neg = 0; neg < 2; neg++) {
+            uint32_t kk[8];
+            if (neg) {  // r - k (k = 0 has no negation to try)
+                bool zero = true;
+                for (int i = 0; i < 8; i++) zero = zero && ks.l[i] == 0;
+                if (zero) continue;
+                uint64_t borrow = 0;
+                for (int i = 0; i < 8; i++) {
+                    const uint64_t d = (uint64_t)FR_R[i] - ks.l[i] - borrow;
+                    kk[i] = (uint32_t)d;
+                    borrow = (d >> 63) & 1;
+                }
+            } else {
+                memcpy(kk, ks.l, 32);
             }
-        } else {
-            memcpy(kk, k, 32);
-        }
-        u128 k2, k1;
-        divmod_lambda(kk, k2, k1);
-        for (int shifted = 0; shifted < 2; shifted++) {
-            // shifted: k1 - lambda < 0 and k2 + 1
-            std::vector<int> d1 = wnaf(shifted ? lam - k1 : k1, W), d2 = wnaf(shifted ? k2 + 1 : k2, W);
-            if (shifted && k1 == 0) continue;
-            if (shifted) for (int &x : d1) x = -x;
-            if (neg) {
-                for (int &x : d1) x = -x;
-                for (int &x : d2) x = -x;
-            }
-            const int c = program_cost(d1, d2);
-            if (best < 0 || c < best) {
-                best = c;
-                b1 = d1;
-                b2 = d2;
+            u128 k2, k1;
+            divmod_lambda(kk, k2, k1);
+            for (int shifted = 0; shifted < 2; shifted++) {
+                // shifted: k1 - lambda < 0 and k2 + 1
+                if (shifted && k1 == 0) continue;
+                std::vector<int> d1 = wnaf(shifted ? lam - k1 : k1, W), d2 = wnaf(shifted ? k2 + 1 : k2, W);
+                if (shifted) for (int &x : d1) x = -x;
+                if (neg) {
+                    for (int &x : d1) x = -x;
+                    for (int &x : d2) x = -x;
+                }
+                const int c = program_cost(d1, d2);
+                if (best < 0 || c < best) {
+                    best = c;
+                    best_t = t;
+                    b1 = d1;
+                    b2 = d2;
+                }
             }
         }
     }
@@ -117,7 +143,9 @@ void append_program(std::vector<uint32_t> &ops, const uint32_t *k, int W) {
             const int digit = t < (int)d.size() ? d[t] : 0;
             if (!digit) continue;
             const uint32_t e = (uint32_t)((digit < 0 ? -digit : digit) - 1) / 2;
-            ops.push_back(pending | (part ? TWOP_PHI : 0u) | (e << TWOP_ENTRY_SHIFT) | (digit < 0 ? TWOP_NEG : 0u));
+            const int power = (best_t + part) % 3;  // phi^power of P
+            const uint32_t phi = power == 1 ? TWOP_PHI : power == 2 ? TWOP_PHI2 : 0u;
+            ops.push_back(pending | phi | (e << TWOP_ENTRY_SHIFT) | (digit < 0 ? TWOP_NEG : 0u));
             pending = 0;
             started = true;
         }
@@ -127,13 +155,9 @@ void append_program(std::vector<uint32_t> &ops, const uint32_t *k, int W) {
 
 // Evaluates a program on the scalar 1 with phi = lambda (Montgomery Fr): must give k.
 bool check_program(const uint32_t *ops, const fs &expect_mont) {
-    fs lam, acc, one;
-    fs_zero(lam);
-    lam.l[0] = (uint32_t)GLV_LAMBDA[0];
-    lam.l[1] = (uint32_t)(GLV_LAMBDA[0] >> 32);
-    lam.l[2] = (uint32_t)GLV_LAMBDA[1];
-    lam.l[3] = (uint32_t)(GLV_LAMBDA[1] >> 32);
-    fs_to_mont(lam, lam);
+    fs lam, lam2, acc, one;
+    lam = lambda_mont();
+    fs_mul(lam2, lam, lam);
     fs_set(one, FR_ONE);
     fs_zero(acc);
     for (int i = 0;; i++) {
@@ -144,7 +168,8 @@ bool check_program(const uint32_t *ops, const fs &expect_mont) {
         fs_zero(m);
         m.l[0] = 2 * ((op >> TWOP_ENTRY_SHIFT) & 7) + 1;
         fs_to_mont(term, m);
-        if (op & TWOP_PHI) fs_mul(term, term, lam);
+        if (op & TWOP_PHI2) fs_mul(term, term, lam2);
+        else if (op & TWOP_PHI) fs_mul(term, term, lam);
         if (op & TWOP_NEG) fs_sub(acc, acc, term);
         else fs_add(acc, acc, term);
     }

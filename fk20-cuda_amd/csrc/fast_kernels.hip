@@ -360,8 +360,9 @@ __device__ __forceinline__ void tw_mul(g1p &x, const uint32_t *ops) {
             T[k] = c;
         }
     }
-    fe beta;
+    fe beta, beta2;
     fe_set(beta, FP_BETA);
+    fe_set(beta2, FP_BETA2);
     g1p acc;
     bool started = false;
 #pragma unroll 1
@@ -373,7 +374,8 @@ __device__ __forceinline__ void tw_mul(g1p &x, const uint32_t *ops) {
         if (op & TWOP_END) break;
         const unsigned e = (op >> TWOP_ENTRY_SHIFT) & (NT - 1);
         g1p q = T[e];
-        if (op & TWOP_PHI) fe_mul(q.x, q.x, beta);
+        if (op & TWOP_PHI2) fe_mul(q.x, q.x, beta2);
+        else if (op & TWOP_PHI) fe_mul(q.x, q.x, beta);
         if (op & TWOP_NEG) fe_neg<16>(q.y, q.y);
         if (started) {
             g1_add(acc, acc, q);

@@ -117,35 +117,39 @@ def _program_cost(d1, d2):
 
 
 def glv_digits(k):
-    """The digit strings (k1, k2) the library's program for k * P uses: of the splits of k
-    and of r - k (then negated), each with 0 <= k1 < lambda or k1 - lambda and k2 + 1, the
-    first of least ladder cost (8 per doubling, 12 per addition)."""
+    """The digit strings (k1, k2) and the phi power t of the library's program for k * P: of
+    the splits of k lambda^-t (t = 0, 1, 2; k1's digits then act on phi^t(P), k2's on
+    phi^(t+1)(P)) and of r minus it (then negated), each with 0 <= k1 < lambda or k1 - lambda
+    and k2 + 1, the first of least ladder cost (8 per doubling, 12 per addition)."""
     best = None
-    for neg in (False, True):
-        if neg and k == 0:
-            continue
-        kk = (R_MOD - k) if neg else k
-        k2, k1 = divmod(kk, GLV_LAMBDA)
-        for shifted in (False, True):
-            if shifted and k1 == 0:
+    lam_inv = pow(GLV_LAMBDA, 2, R_MOD)  # lambda^3 = 1
+    for t in range(3):
+        kt = k * pow(lam_inv, t, R_MOD) % R_MOD
+        for neg in (False, True):
+            if neg and kt == 0:
                 continue
-            d1 = _wnaf(GLV_LAMBDA - k1 if shifted else k1, WNAF_W)
-            d2 = _wnaf(k2 + 1 if shifted else k2, WNAF_W)
-            if shifted:
-                d1 = [-x for x in d1]
-            if neg:
-                d1, d2 = [-x for x in d1], [-x for x in d2]
-            c = _program_cost(d1, d2)
-            if best is None or c < best[0]:
-                best = (c, d1, d2)
-    return best[1], best[2]
+            kk = (R_MOD - kt) if neg else kt
+            k2, k1 = divmod(kk, GLV_LAMBDA)
+            for shifted in (False, True):
+                if shifted and k1 == 0:
+                    continue
+                d1 = _wnaf(GLV_LAMBDA - k1 if shifted else k1, WNAF_W)
+                d2 = _wnaf(k2 + 1 if shifted else k2, WNAF_W)
+                if shifted:
+                    d1 = [-x for x in d1]
+                if neg:
+                    d1, d2 = [-x for x in d1], [-x for x in d2]
+                c = _program_cost(d1, d2)
+                if best is None or c < best[0]:
+                    best = (c, d1, d2, t)
+    return best[1], best[2], best[3]
 
 
 def twiddle_program(e_idx, inverse=False):
     """(doublings, additions, phi_muls, ops) of the program for w^e (w of order 512)."""
     w = pow(5, (R_MOD - 1) // 512, R_MOD)
     k = pow(w, (512 - e_idx) % 512 if inverse else e_idx, R_MOD)
-    d1, d2 = glv_digits(k)
+    d1, d2, tphi = glv_digits(k)
     L = max(len(d1), len(d2))
     dbl = adds = phi = ops = 0
     started = False
@@ -156,7 +160,7 @@ def twiddle_program(e_idx, inverse=False):
             x = d[t] if t < len(d) else 0
             if x:
                 ops += 1
-                phi += part
+                phi += (tphi + part) % 3 != 0  # one Fp mul (beta or beta^2) per such digit
                 if started:
                     adds += 1
                 started = True
