@@ -222,6 +222,25 @@ def time_p1(b, sample_rows):
     return lat
 
 
+def time_f1(b, prover, sid, reps):
+    """Single-row latency of the batched path (C1 on the GPU): fk20_prove_rows with rows = 1."""
+    stream = L.Stream()
+    e0, e1 = L.Event(), L.Event()
+    prover.prove_device(sid, b.h_fft, b.poly, 1, stream=stream.handle)
+    stream.synchronize()
+    lat = []
+    for _ in range(reps):
+        e0.record(stream)
+        prover.prove_device(sid, b.h_fft, b.poly, 1, stream=stream.handle)
+        e1.record(stream)
+        lat.append(e0.elapsed_ms(e1))
+    stream.synchronize()
+    e0.destroy()
+    e1.destroy()
+    stream.destroy()
+    return lat
+
+
 # ---------------------------------------------------------------- C4: uneven batch
 
 def time_uneven(prover, rows_each, window, steps, seed):
@@ -659,6 +678,9 @@ def main():
         forms["F_over_P1"] = round(value / forms["P1_rows_per_s"], 1)
     if "P2_rows_per_s" in forms:
         forms["F_over_P2"] = round(value / forms["P2_rows_per_s"], 2)
+    if args.p1_rows:
+        f1 = time_f1(b, prover, sid, max(3, args.p1_rows))
+        forms["F_1row_latency_ms_p50"] = round(statistics.median(f1), 3)
     uneven = None
     if args.uneven_rows:
         log("timing uneven batch: %d rows each of N=1024/2048/4096 in one pass" % args.uneven_rows)
