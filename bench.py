@@ -360,6 +360,41 @@ def bench_micro(args):
                       "kernels": out}))
 
 
+# ---------------------------------------------------------------- field multiplication rate
+
+def bench_fieldmul(args):
+    """Throughput of the Fp Montgomery multiplication itself (14 x 29-bit limbs), the unit
+    every G1 kernel is built from: Fermat inversions a^(p-2) through fk20_fp_op (one lane per
+    element, 384 squarings + one multiplication per set bit of p - 2, no memory traffic in
+    the loop), on `fieldmul_count` uniform non-zero elements; a sample is checked on the host."""
+    count = args.fieldmul_count
+    p_mod = 0x1A0111EA397FE69A4B1BA7B6434BACD764774B84F38512BF6730D2A0F6B0F6241EABFFFEB153FFFFB9FEFFFFFFFFAAAB
+    ops_per = 384 + bin(p_mod - 2).count("1")
+    a = S.random_fp_nonzero(count, args.seed + 7)
+    A = L.DeviceBuffer.from_array(a)
+    Z = L.DeviceBuffer(count * 48)
+    stream = L.Stream()
+    L.call("fk20_fp_op", 5, Z.ptr, A.ptr, None, count, stream.handle)
+    stream.synchronize()
+    ms = statistics.median(time_calls(lambda: L.call("fk20_fp_op", 5, Z.ptr, A.ptr, None, count, stream.handle),
+                                      args.steps, stream))
+    z = Z.download((count, 6))
+    for i in range(0, count, max(1, count // 16)):
+        x = sum(int(w) << (64 * k) for k, w in enumerate(a[i].tolist()))
+        y = sum(int(w) << (64 * k) for k, w in enumerate(z[i].tolist()))
+        assert x * y % p_mod == 1, i
+    for b_ in (A, Z):
+        b_.free()
+    stream.destroy()
+    rate = count * ops_per / (ms / 1e3)
+    print(json.dumps({"metric": "Fp multiplications per second (14x29-bit Montgomery, in Fermat inversions)",
+                      "value": round(rate, 1), "unit": "Fp mul/s", "n_gpus": 1, "steps": args.steps,
+                      "higher_is_better": True, "dtype": "u32", "data": "synthetic: uniform non-zero Fp",
+                      "config": {"workload": "fp_mul_rate", "elements": count, "muls_per_element": ops_per},
+                      "ms_p50": round(ms, 3), "mul32_per_s": round(rate * WL.FP_MUL32, 1),
+                      "note": "squarings counted as multiplications; mul32 = the 300-per-Fp-mul convention"}))
+
+
 # ---------------------------------------------------------------- commitments (the step before the path)
 
 def bench_commit(args):
@@ -550,7 +585,8 @@ def main():
                     help="transform stages as one persistent dependency-driven launch (default: one launch per stage)")
     ap.add_argument("--uneven-rows", type=int, default=512,
                     help="rows per size in the uneven N=1024/2048/4096 batch (0: skip)")
-    ap.add_argument("--workload", default="batch", choices=["batch", "serve", "micro", "commit"])
+    ap.add_argument("--workload", default="batch", choices=["batch", "serve", "micro", "commit", "fieldmul"])
+    ap.add_argument("--fieldmul-count", type=int, default=1 << 21, help="fieldmul: elements inverted per call")
     ap.add_argument("--calibrate-copy-mb", type=int, default=0,
                     help="before timing, one device-to-device copy of this many MiB (the known-size "
                          "kernel that scales FETCH_SIZE / WRITE_SIZE counter runs)")
@@ -580,6 +616,9 @@ def main():
         return
     if args.workload == "commit":
         bench_commit(args)
+        return
+    if args.workload == "fieldmul":
+        bench_fieldmul(args)
         return
 
     if args.calibrate_copy_mb:
