@@ -42,6 +42,7 @@ def log(msg):
 
 
 TRAFFIC_JSON = os.path.join(ROOT, "profiles", "fk20_batch_hbm_traffic.json")
+FPMUL_JSON = os.path.join(ROOT, "profiles", "fp_mul_rate.json")
 # launch kind (as timed by the library) -> kernel symbol prefix in rocprofv3 output
 KIND_SYMBOL = {"k_dit_stage": "fk20::k_dit_multi<", "k_dif_stage": "fk20::k_dif_multi<",
                "k_middle": "fk20::k_middle_multi<", "k_msm_comb": "fk20::k_msm_multi",
@@ -690,6 +691,13 @@ def main():
     }
 
     roofline["valu_counters"] = measured_valu(dom, args.N, args.rows, args.window, avg_ms / 1e3)
+    try:  # the library's own Fp multiplication rate (bench.py --workload fieldmul)
+        with open(FPMUL_JSON) as f:
+            ceil = json.load(f)["value"]
+        roofline["fp_mul_ceiling"] = {"fp_mul_per_s": ceil, "frac": round(exe_fp / (avg_ms / 1e3) / ceil, 4),
+                                      "source": os.path.relpath(FPMUL_JSON, ROOT)}
+    except (OSError, KeyError, ValueError):
+        roofline["fp_mul_ceiling"] = None
     tr = measured_traffic(dom, args.N, args.rows, args.window)
     if tr:
         roofline["traffic"] = tr["bytes"]
