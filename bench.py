@@ -51,8 +51,10 @@ KIND_SYMBOL = {"k_dit_stage": "fk20::k_dit_multi<", "k_dif_stage": "fk20::k_dif_
 
 
 def _kernel_entry(kernels, kind):
-    prefix = KIND_SYMBOL.get(kind)
-    hits = [v for k, v in kernels.items() if prefix and k.startswith(prefix)]
+    """The profile entry of a launch kind: exact kernel name, or any instance of a template
+    (symbols ending in '<')."""
+    sym = KIND_SYMBOL.get(kind)
+    hits = [v for k, v in kernels.items() if sym and (k == sym or (sym.endswith("<") and k.startswith(sym)))]
     return hits[0] if len(hits) == 1 else None
 
 

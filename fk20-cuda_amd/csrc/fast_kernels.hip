@@ -287,6 +287,43 @@ __global__ void __launch_bounds__(64) k_msm_multi(Launch L) {
     pt_store(it.D + ((size_t)bitrev(j, it.logn2) * rows + row) * 36, acc);
 }
 
+// The same MSM for launches whose items all have at most MSM_SMALL_ROWS rows, where the
+// 16 x ceil(256/w) mixed additions of a lane are latency: lanes are (term i, row) pairs, 16
+// terms x 4 rows per wave, each lane sums one term's windows, and the 16 partial sums of a
+// row are added in LDS (4 levels).  Output to D[bitrev(j)][row] like k_msm_multi.
+static constexpr unsigned MSM_SMALL_ROWS = 16;
+
+__global__ void __launch_bounds__(64) k_msm_multi_small(Launch L) {
+    __shared__ __attribute__((aligned(16))) uint32_t lds[64 * 36];
+    unsigned local;
+    const Item &it = L.item[find_item(L, blockIdx.x, local)];
+    const unsigned rows = it.rows, n2 = 1u << it.logn2, rg4 = (rows + 3) / 4;
+    const unsigned WIN = it.win, NWIN = (256 + WIN - 1) / WIN, H = 1u << (WIN - 1);
+    const unsigned j = local / rg4, t = threadIdx.x, i = t % 16, r = t / 16;
+    const unsigned row0 = (local % rg4) * 4 + r, row = min(row0, rows - 1);  // spare lanes repeat a row
+    const unsigned b = i * n2 + j;
+    g1p acc;
+    g1_inf(acc);
+    if (!it.inf[b]) {
+        uint32_t k[8];
+        ld_words(k, it.tcT + (((size_t)j * 16 + i) * rows + row) * FR_WORDS, 2);
+        window_term(acc, k, it.table + (size_t)b * NWIN * H * G1A_WORDS, WIN);
+    }
+    pt_store(lds + t * 36, acc);
+#pragma unroll 1
+    for (unsigned h = 8; h >= 1; h /= 2) {
+        __syncthreads();
+        g1p o;
+        if (i < h) {
+            pt_load(o, lds + (t + h) * 36);
+            g1_add(acc, acc, o);
+        }
+        __syncthreads();
+        if (i < h) pt_store(lds + t * 36, acc);
+    }
+    if (i == 0 && row0 < rows) pt_store(it.D + ((size_t)bitrev(j, it.logn2) * rows + row) * 36, acc);
+}
+
 // ---------------------------------------------------------------- KZG commitments
 //
 // commit[row] = sum_i poly[row][i] * srs[i] (FK20Py kzg_proofs.py:159 commit_to_poly) over
@@ -691,9 +728,13 @@ hipError_t launch_batch(const Launch &base, const DevTables *T, const uint32_t *
     if (total) hipLaunchKernelGGL(k_toeplitz_fft_multi, dim3(total), dim3(256), 0, s, L, T);
     hook(LK_TOEPLITZ_FFT);
     // MSM
-    for (unsigned k = 0; k < L.nitems; k++) per[k] = (1u << L.item[k].logn2) * ((L.item[k].rows + 63) / 64);
+    bool small = true;
+    for (unsigned k = 0; k < L.nitems; k++) small = small && L.item[k].rows <= MSM_SMALL_ROWS;
+    for (unsigned k = 0; k < L.nitems; k++)
+        per[k] = (1u << L.item[k].logn2) * (small ? (L.item[k].rows + 3) / 4 : (L.item[k].rows + 63) / 64);
     total = set_blocks(L, per);
-    if (total) hipLaunchKernelGGL(k_msm_multi, dim3(total), dim3(64), 0, s, L);
+    if (total && small) hipLaunchKernelGGL(k_msm_multi_small, dim3(total), dim3(64), 0, s, L);
+    else if (total) hipLaunchKernelGGL(k_msm_multi, dim3(total), dim3(64), 0, s, L);
     hook(LK_MSM);
     // transform stages: global stage t = 0 .. 2*lmax - 2; item k (L = logn2) runs local stage
     // u = t - (lmax - L) while 0 <= u <= 2L - 2 (DIT right-aligned to the middle at t = lmax-1,

@@ -183,6 +183,65 @@ __global__ void __launch_bounds__(256, 2) k_msm_booth(uint32_t *out, const int8_
     }
 }
 
+// The same MSM for a few rows (at most MSM_SMALL_ROWS), where latency counts: a lane per
+// (term, row), 16 terms x 4 rows per wave, so each lane runs one term's ladder (42 windows
+// of 6 doublings, 43 additions) instead of all 16 terms', and the 16 partial sums of a row
+// are added in LDS (4 levels).  Lane (i, r) builds the multiples (r+1) X_i + 4k X_i of its
+// term's base into LDS first.
+static constexpr unsigned MSM_SMALL_ROWS = 16;
+
+__global__ void __launch_bounds__(64) k_msm_booth_small(uint32_t *out, const int8_t *digits, const uint32_t *xpk,
+                                                        unsigned n2, unsigned rows, unsigned to_D) {
+    __shared__ __attribute__((aligned(16))) uint32_t lds[16 * MSM_HALF * 36];
+    const unsigned j = blockIdx.x, t = threadIdx.x, i = t % 16, r = t / 16;
+    {
+        g1p X, step, e;
+        g1_lds_load(X, xpk + ((size_t)i * n2 + j) * 36);  // packed Montgomery points in HBM
+        g1_dbl(step, X);                                  // 2 X
+        if (r == 0) e = X;
+        else if (r == 1) e = step;
+        else if (r == 2) g1_add(e, step, X);
+        else g1_dbl(e, step);
+        g1_dbl(step, step);                               // 4 X
+#pragma unroll 1
+        for (unsigned k = 0; k < MSM_HALF / 4; k++) {     // entry m - 1 holds m X_i, m = r + 1 + 4k
+            g1_lds_store(lds + (i * MSM_HALF + r + 4 * k) * 36, e);
+            if (k + 1 < MSM_HALF / 4) g1_add(e, e, step);
+        }
+    }
+    __syncthreads();
+    const unsigned row0 = blockIdx.y * 4 + r;
+    const unsigned row = min(row0, rows - 1);  // lanes past the last row repeat it
+    const int8_t *dg = digits + ((size_t)j * 16 + i) * MSM_NWIN * rows + row;
+    g1p acc;
+    g1_inf(acc);
+#pragma unroll 1
+    for (int w = MSM_NWIN - 1; w >= 0; w--) {
+        if (w != (int)MSM_NWIN - 1) {
+#pragma unroll 1
+            for (int d = 0; d < 6; d++) g1_dbl(acc, acc);
+        }
+        msm_booth_add(acc, lds + i * MSM_HALF * 36, dg[(size_t)w * rows]);
+    }
+    __syncthreads();  // the table is dead: its first 64 slots take the partial sums
+    g1_lds_store(lds + t * 36, acc);
+#pragma unroll 1
+    for (unsigned h = 8; h >= 1; h /= 2) {
+        __syncthreads();
+        g1p o;
+        if (i < h) {
+            g1_lds_load(o, lds + (t + h) * 36);
+            g1_add(acc, acc, o);
+        }
+        __syncthreads();
+        if (i < h) g1_lds_store(lds + t * 36, acc);
+    }
+    if (i == 0 && row0 < rows) {
+        if (to_D) g1_lds_store(out + ((size_t)bitrev(j, 31 - __builtin_clz(n2)) * rows + row) * 36, acc);
+        else g1_gstore_std(out + ((size_t)row * n2 + j) * G1P_WORDS, acc);
+    }
+}
+
 // xext[i][m] = srs[N-17-i-16m] for m <= k-2, infinity otherwise (FK20Py fk20_multi.py:82-85:
 // setup[n-l-1-i::-l] + [Z1], padded to 2k with Z1).
 __global__ void k_setup_gather(uint32_t *xext, const uint32_t *srs, unsigned N) {
@@ -372,6 +431,11 @@ static hipError_t msm_plain(uint32_t *out, const uint32_t *tc, const uint32_t *x
     if (!rows) return hipSuccess;
     const size_t scalars = (size_t)rows * n2 * 16;
     hipLaunchKernelGGL(k_msm_digits, dim3((unsigned)((scalars + 255) / 256)), dim3(256), 0, s, digits, tc, n2, rows, m);
+    if (rows <= MSM_SMALL_ROWS) {
+        hipLaunchKernelGGL(k_msm_booth_small, dim3(n2, (rows + 3) / 4), dim3(64), 0, s, out, (const int8_t *)digits,
+                           xpk, n2, rows, m.to_D);
+        return hipGetLastError();
+    }
     const unsigned threads = 64 * std::min(4u, (rows + 63) / 64);
     hipLaunchKernelGGL(k_msm_booth, dim3(n2, (rows + threads - 1) / threads), dim3(threads), 0, s, out,
                        (const int8_t *)digits, xpk, n2, rows, m.to_D);

@@ -125,6 +125,27 @@ def test_uneven_batch_one_pass(prover):
         prover.unload_setup(sid)
 
 
+def test_uneven_small_batch_one_pass(prover):
+    """Items of at most 16 rows (the MSM's term-parallel kernel) across three sizes, against
+    the closed form row by row and a large batch of the same rows."""
+    groups, taus = [], {}
+    for N, rows, seed in ((1024, 1, 311), (2048, 7, 312), (4096, 16, 313)):
+        tau = S.random_secret(seed)
+        sid = prover.load_setup(O.xext_fft_from_tau(tau, N), N, 8)
+        groups.append((sid, S.random_polys(rows, N, seed + 10)))
+        taus[sid] = tau
+    outs = prover.prove_batch(groups)
+    for (sid, poly), got in zip(groups, outs):
+        for r in range(poly.shape[0]):
+            assert np.array_equal(aff(got[r]), aff(shadow_points(poly[r], taus[sid]))), (sid, r)
+    # the 16-row group inside a 17-row batch takes the row-parallel kernel: same points
+    sid, poly = groups[2]
+    big = prover.prove(sid, np.concatenate([poly, poly[:1]]))
+    assert np.array_equal(aff(big[:16]), aff(outs[2]))
+    for sid in list(taus):
+        prover.unload_setup(sid)
+
+
 def test_chain_equals_per_stage_launches(prover):
     """The persistent transform chain and one launch per stage (default) give bit-identical
     projective outputs, on an uneven batch with partial 64-row groups; the chain also matches
