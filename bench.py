@@ -43,6 +43,7 @@ def log(msg):
 
 TRAFFIC_JSON = os.path.join(ROOT, "profiles", "fk20_batch_hbm_traffic.json")
 FPMUL_JSON = os.path.join(ROOT, "profiles", "fp_mul_rate.json")
+MAD_JSON = os.path.join(ROOT, "profiles", "mad_rate.json")
 # launch kind (as timed by the library) -> kernel symbol prefix in rocprofv3 output
 KIND_SYMBOL = {"k_dit_stage": "fk20::k_dit_multi<", "k_dif_stage": "fk20::k_dif_multi<",
                "k_middle": "fk20::k_middle_multi<", "k_msm_comb": "fk20::k_msm_multi",
@@ -365,6 +366,29 @@ def bench_micro(args):
 
 # ---------------------------------------------------------------- field multiplication rate
 
+def bench_madrate(args):
+    """Issue rate of v_mad_u64_u32 through fk20_mad_probe: 8 blocks of 256 lanes per CU
+    (8 waves per SIMD), 8 independent chains per lane."""
+    blocks, iters = 256 * 8, 20000
+    out = L.DeviceBuffer(blocks * 256 * 8)
+    stream = L.Stream()
+    L.call("fk20_mad_probe", out.ptr, blocks, 100, stream.handle)
+    stream.synchronize()
+    ms = statistics.median(time_calls(lambda: L.call("fk20_mad_probe", out.ptr, blocks, iters, stream.handle),
+                                      args.steps, stream))
+    out.free()
+    stream.destroy()
+    mads = blocks * 256 * 8 * iters
+    rate = mads / (ms / 1e3)
+    print(json.dumps({"metric": "v_mad_u64_u32 issue rate (fk20_mad_probe)", "value": round(rate, 1),
+                      "unit": "mad/s", "n_gpus": 1, "steps": args.steps, "higher_is_better": True,
+                      "dtype": "u32", "data": "synthetic",
+                      "config": {"workload": "mad_rate", "blocks": blocks, "lanes_per_block": 256,
+                                 "chains_per_lane": 8, "iters": iters},
+                      "ms_p50": round(ms, 3), "peak_assumed": WL.PEAK_MUL32_PER_S,
+                      "frac_of_assumed_peak": round(rate / WL.PEAK_MUL32_PER_S, 4)}))
+
+
 def bench_fieldmul(args):
     """Throughput of the Fp Montgomery multiplication itself (14 x 29-bit limbs), the unit
     every G1 kernel is built from: Fermat inversions a^(p-2) through fk20_fp_op (one lane per
@@ -588,7 +612,7 @@ def main():
                     help="transform stages as one persistent dependency-driven launch (default: one launch per stage)")
     ap.add_argument("--uneven-rows", type=int, default=512,
                     help="rows per size in the uneven N=1024/2048/4096 batch (0: skip)")
-    ap.add_argument("--workload", default="batch", choices=["batch", "serve", "micro", "commit", "fieldmul"])
+    ap.add_argument("--workload", default="batch", choices=["batch", "serve", "micro", "commit", "fieldmul", "madrate"])
     ap.add_argument("--fieldmul-count", type=int, default=1 << 21, help="fieldmul: elements inverted per call")
     ap.add_argument("--calibrate-copy-mb", type=int, default=0,
                     help="before timing, one device-to-device copy of this many MiB (the known-size "
@@ -622,6 +646,9 @@ def main():
         return
     if args.workload == "fieldmul":
         bench_fieldmul(args)
+        return
+    if args.workload == "madrate":
+        bench_madrate(args)
         return
 
     if args.calibrate_copy_mb:
@@ -700,6 +727,14 @@ def main():
                                       "source": os.path.relpath(FPMUL_JSON, ROOT)}
     except (OSError, KeyError, ValueError):
         roofline["fp_mul_ceiling"] = None
+    try:  # the measured v_mad_u64_u32 issue rate (bench.py --workload madrate)
+        with open(MAD_JSON) as f:
+            mad = json.load(f)["value"]
+        roofline["mad_probe"] = {"mad_per_s": mad, "peak_frac": round(mad / WL.PEAK_MUL32_PER_S, 4),
+                                 "achieved_frac": round(achieved * 1e12 / mad, 4),
+                                 "source": os.path.relpath(MAD_JSON, ROOT)}
+    except (OSError, KeyError, ValueError):
+        roofline["mad_probe"] = None
     tr = measured_traffic(dom, args.N, args.rows, args.window)
     if tr:
         roofline["traffic"] = tr["bytes"]

@@ -322,6 +322,11 @@ int fk20_setup2xext_fft(fk20_g1p_t *xext_fft, const fk20_g1p_t *setup, unsigned 
 
 // ---------------------------------------------------------------- helpers
 
+int fk20_mad_probe(uint64_t *out, unsigned blocks, unsigned iters, fk20_stream_t stream) {
+    if (!out || !blocks) return FK20_EINVAL;
+    return hip_status(launch_mad_probe(out, blocks, iters, S(stream)));
+}
+
 int fk20_g1p_to_affine(fk20_g1a_t *out, const fk20_g1p_t *in, size_t count, fk20_stream_t stream) {
     if (count && (!out || !in)) return FK20_EINVAL;
     return hip_status(launch_to_affine(W(out), W(in), count, S(stream)));

@@ -58,6 +58,8 @@ hipError_t launch_msm_plain(uint32_t *hext_fft, const uint32_t *tc_fft, const ui
 // Montgomery), the layouts of the column-batched transforms.
 hipError_t launch_msm_plain_T(uint32_t *D, const uint32_t *tcT, const uint32_t *xext_packed, int8_t *digits,
                               unsigned rows, unsigned n2, hipStream_t s);
+// blocks x 256 lanes, each 8 x iters v_mad_u64_u32; out receives blocks * 256 words.
+hipError_t launch_mad_probe(uint64_t *out, unsigned blocks, unsigned iters, hipStream_t s);
 hipError_t launch_setup_gather(uint32_t *xext, const uint32_t *srs, unsigned N, hipStream_t s);
 hipError_t launch_to_affine(uint32_t *out, const uint32_t *in, size_t count, hipStream_t s);
 hipError_t launch_compress(uint32_t *out, const uint32_t *in, size_t count, hipStream_t s);

@@ -242,6 +242,28 @@ __global__ void __launch_bounds__(64) k_msm_booth_small(uint32_t *out, const int
     }
 }
 
+// Issue-rate probe of the 32 x 32 -> 64-bit multiply-add every Fp product is built from: 8
+// independent accumulator chains per lane, each step acc_k = x_k * lo(acc_(k+1)) + acc_k,
+// so every instruction is one v_mad_u64_u32 whose operands change each iteration.
+__global__ void __launch_bounds__(256) k_mad_probe(uint64_t *out, unsigned iters) {
+    uint64_t acc[8];
+    uint32_t x[8];
+#pragma unroll
+    for (int k = 0; k < 8; k++) {
+        acc[k] = (uint64_t)(threadIdx.x + 977u * k) << 7 | k;
+        x[k] = 0x9e3779b9u * (blockIdx.x + 1) + 0x85ebca6bu * k;
+    }
+#pragma unroll 1
+    for (unsigned it = 0; it < iters; it++) {
+#pragma unroll
+        for (int k = 0; k < 8; k++) acc[k] = (uint64_t)x[k] * (uint32_t)acc[(k + 1) & 7] + acc[k];
+    }
+    uint64_t h = 0;
+#pragma unroll
+    for (int k = 0; k < 8; k++) h ^= acc[k];
+    out[(size_t)blockIdx.x * 256 + threadIdx.x] = h;
+}
+
 // xext[i][m] = srs[N-17-i-16m] for m <= k-2, infinity otherwise (FK20Py fk20_multi.py:82-85:
 // setup[n-l-1-i::-l] + [Z1], padded to 2k with Z1).
 __global__ void k_setup_gather(uint32_t *xext, const uint32_t *srs, unsigned N) {
@@ -452,6 +474,11 @@ hipError_t launch_msm_plain_T(uint32_t *D, const uint32_t *tcT, const uint32_t *
                               unsigned n2, hipStream_t s) {
     // scalars tcT[j][i][row], output D[bitrev(j)][row] (packed Montgomery)
     return msm_plain(D, tcT, xpk, digits, rows, n2, MsmLayout{1u, rows, 16u * rows, 1u}, s);
+}
+
+hipError_t launch_mad_probe(uint64_t *out, unsigned blocks, unsigned iters, hipStream_t s) {
+    hipLaunchKernelGGL(k_mad_probe, dim3(blocks), dim3(256), 0, s, out, iters);
+    return hipGetLastError();
 }
 
 hipError_t launch_setup_gather(uint32_t *xext, const uint32_t *srs, unsigned N, hipStream_t s) {

@@ -79,6 +79,7 @@ _SIGS = {
     "fk20_host_alloc": ([ctypes.POINTER(_P), _SZ], _I),
     "fk20_host_free": ([_P], _I),
     "fk20_twiddle_selfcheck": ([ctypes.POINTER(_U), ctypes.POINTER(_U)], _I),
+    "fk20_mad_probe": ([_P, _U, _U, _P], _I),
     "fk20_ctx_set_launch_timing": ([_P, _I], _I),
     "fk20_ctx_launch_times": ([_P, ctypes.POINTER(ctypes.c_float), ctypes.POINTER(_I), _I, ctypes.POINTER(_I)], _I),
     "fk20_status_string": ([_I], ctypes.c_char_p),

@@ -241,6 +241,12 @@ int fk20_ctx_launch_times(fk20_ctx_t *ctx, float *ms, int *kinds, int max, int *
  * the programs (and the transform kernels' tables of odd multiples) use.  Needs no device. */
 int fk20_twiddle_selfcheck(unsigned *total_ops, unsigned *wnaf_width);
 
+/* Measurement helper: blocks x 256 lanes each issue 8 x iters dependent-free 32 x 32 -> 64-bit
+ * multiply-adds (v_mad_u64_u32, the instruction every Fp product is built from); out receives
+ * blocks * 256 words that depend on every result.  Time it to obtain the device's
+ * multiply-add issue rate.  No reference counterpart. */
+int fk20_mad_probe(uint64_t *out, unsigned blocks, unsigned iters, fk20_stream_t stream);
+
 /* ------------------------------------------------------------------ helpers */
 
 /* Canonical affine form of count points.  Replaces g1a_fromG1p_wrapper (g1.cuh:98; g1p.cu:168). */
