@@ -51,6 +51,8 @@ def test_binding_covers_header():
     lambda l: l.fk20_fp_op(9, 1, 1, 1, 1, None),
     lambda l: l.fk20_fr_op(0, 1, 1, None, 1, None),
     lambda l: l.fk20_g1p_op(3, 1, 1, None, None, 1, None),
+    lambda l: l.fk20_mad_probe(None, 1, 1, None),
+    lambda l: l.fk20_mad_probe(1, 0, 1, None),
 ])
 def test_invalid_arguments_rejected(call):
     assert call(L.lib()) == -1  # FK20_EINVAL
