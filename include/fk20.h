@@ -127,7 +127,8 @@ int fk20_ctx_set_graph(fk20_ctx_t *ctx, int enable);
 /* 1: the G1 IFFT / zero / FFT stages of a batch run as one persistent launch in which each
  * 64-row butterfly starts as soon as the two butterflies it reads have finished; 0 (default):
  * one launch per stage.  Results are identical; on MI355X the per-stage launches are faster
- * (the persistent form pays agent-scope cache write-back/invalidate around every butterfly). */
+ * (in the persistent form a wave that has taken a butterfly holds its slot while it waits for
+ * the butterfly's inputs, so fewer ladders run at once). */
 int fk20_ctx_set_chain(fk20_ctx_t *ctx, int enable);
 /* 2 (default): fk20_prove_rows / fk20_prove_batch split the rows into two halves at a 64-row
  * boundary and run the halves' launch sequences on two internal streams forked from and
