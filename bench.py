@@ -48,7 +48,7 @@ MAD_JSON = os.path.join(ROOT, "profiles", "mad_rate.json")
 KIND_SYMBOL = {"k_dit_stage": "fk20::k_dit_multi<", "k_dif_stage": "fk20::k_dif_multi<",
                "k_middle": "fk20::k_middle_multi<", "k_msm_comb": "fk20::k_msm_multi",
                "k_toeplitz_fft_T": "fk20::k_toeplitz_fft_multi", "k_fft_output": "fk20::k_output_multi",
-               "k_transform_chain": "fk20::k_chain<"}
+               "k_transform_chain": "fk20::k_chain<", "k_twiddle_layer": "fk20::k_twiddle_multi<"}
 
 
 def _kernel_entry(kernels, kind):
@@ -688,6 +688,8 @@ def main():
         stages = list(range(lg - 1))
     elif dom == "k_dif_stage":
         stages = list(range(1, lg))
+    elif dom == "k_twiddle_layer":
+        stages = WL.r22_layers(args.N)
     else:
         stages = [None]
     # executed work of this kernel's algorithm per launch (mean over its launches in a step)
@@ -742,6 +744,9 @@ def main():
         n2 = args.N // 8
         alg = {"k_dit_stage": 2, "k_dif_stage": 2, "k_middle": 2,
                "k_transform_chain": 2 * (2 * lg - 1)}.get(dom, 0) * n2 * args.rows * 144
+        if dom == "k_twiddle_layer":  # each twiddled point read and written once
+            layers = WL.r22_layers(args.N)
+            alg = round(sum(len(WL.r22_layer(sd, lg)) for sd, _ in layers) / len(layers) * args.rows * 288)
         tr["algorithmic_bytes"] = alg or None
         roofline["traffic_detail"] = tr
 

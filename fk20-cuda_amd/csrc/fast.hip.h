@@ -89,7 +89,10 @@ struct Chain {
 };
 
 // Kernel kinds of the batched path, reported by the per-launch timing of the context.
-enum LaunchKind { LK_TOEPLITZ_FFT = 0, LK_MSM = 1, LK_DIT = 2, LK_MIDDLE = 3, LK_DIF = 4, LK_OUTPUT = 5, LK_CHAIN = 6 };
+enum LaunchKind {
+    LK_TOEPLITZ_FFT = 0, LK_MSM = 1, LK_DIT = 2, LK_MIDDLE = 3, LK_DIF = 4, LK_OUTPUT = 5, LK_CHAIN = 6,
+    LK_TWIDDLE = 7
+};
 // Called after each launch when the context times launches (fn == nullptr otherwise).
 struct LaunchHook {
     void (*fn)(void *arg, int kind);

@@ -10,12 +10,14 @@
 //                      signed fixed windows: ceil(256/w) mixed additions per term, no
 //                      doublings.  Lanes = rows of one column, so a wave gathers from the
 //                      table region of one base.  Output goes to D[bitrev(j)][row].
-//  5. k_dit_stage / k_middle / k_dif_stage
+//  5. k_dit_stage / k_middle / k_dif_stage / k_twiddle_multi
 //                      the G1 IFFT (DIT, bit-reversed in, natural out), the fused last IFFT
 //                      stage + zero upper half + first FFT stage, and the G1 FFT (DIF,
 //                      natural in, bit-reversed out), column-batched: a wave is one
-//                      butterfly for 64 rows, so the twiddle is wave-uniform and runs as a
-//                      precomputed GLV + wNAF program (uniform control flow, no divergence).
+//                      butterfly (or, in a twiddle layer, one point) for 64 rows, so the
+//                      twiddle is wave-uniform and runs as a precomputed GLV + wNAF program
+//                      (uniform control flow, no divergence).  Radix-2^2 schedule: the
+//                      butterflies carry only w^+-128 twiddles, the point-wise layers the rest.
 //  6. k_fft_output     D[pos][row] -> h_fft[row][bitrev(pos)] in the reference ABI.
 // Same mathematics as fk20_msm.cu:23-66, g1p_fft.cu:25-330 and fk20_hext_fft2h_fft.cu:27-70;
 // the layout and algorithms are this library's own.
@@ -272,8 +274,7 @@ __global__ void __launch_bounds__(64) k_msm_multi(Launch L) {
     const unsigned rows = it.rows, n2 = 1u << it.logn2, rg = (rows + 63) / 64;
     const unsigned WIN = it.win, NWIN = (256 + WIN - 1) / WIN, H = 1u << (WIN - 1);
     const unsigned j = local / rg;
-    const unsigned row = (local % rg) * 64 + threadIdx.x;
-    if (row >= rows) return;
+    const unsigned row = min((local % rg) * 64 + threadIdx.x, rows - 1);  // spare lanes repeat the last row
     g1p acc;
     g1_inf(acc);
 #pragma unroll 1
@@ -335,8 +336,7 @@ __global__ void __launch_bounds__(64) k_commit_partial(const uint32_t *poly, con
                                                        const uint32_t *inf, uint32_t *partial, unsigned rows,
                                                        unsigned N, unsigned win_bits, unsigned chunk) {
     const unsigned rg = (rows + 63) / 64, c = blockIdx.x / rg;
-    const unsigned row = (blockIdx.x % rg) * 64 + threadIdx.x;
-    if (row >= rows) return;
+    const unsigned row = min((blockIdx.x % rg) * 64 + threadIdx.x, rows - 1);  // spare lanes repeat the last row
     const unsigned NWIN = (256 + win_bits - 1) / win_bits, H = 1u << (win_bits - 1);
     const unsigned i0 = c * chunk, i1 = min(i0 + chunk, N);
     g1p acc;
@@ -440,24 +440,64 @@ struct Butterfly {
     unsigned idx;     // twiddle exponent (w^+-idx), 0: none
 };
 
+// R22 = false: radix 2, every butterfly carries its stage's twiddle (the persistent chain).
+// R22 = true: radix 2^2 (the per-stage launches).  With the FFT's DIF stages numbered
+// s = 0 .. L-1 (the IFFT's DIT stage u is the transpose of DIF stage L-1-u), stages pair up
+// as (0, 1), (2, 3), ..: the even stage of a pair keeps only the w^(n/4) part (w^+-128 here,
+// a 64-bit GLV program) of its twiddle, on the half of its butterflies whose offset in the
+// block is at least half the span, and defers the rest to the odd stage, whose butterflies
+// carry no twiddle; a separate point-wise layer (k_twiddle_multi, layer_twiddle) then applies
+// the deferred and own twiddles of the odd stage, three per four points of each block.  A
+// lone last DIF stage (L odd) has only trivial twiddles.  Both schedules compute the same
+// transform exactly.
+template <bool R22>
 __device__ __forceinline__ Butterfly stage_positions(unsigned u, unsigned logn2, unsigned b) {
     Butterfly f;
     if (u + 1 < logn2) {  // IFFT DIT stage s = u: span 2^s
         const unsigned s = u, j = b & ((1u << s) - 1);
         f.p0 = ((b >> s) << (s + 1)) + j;
         f.p1 = f.p0 + (1u << s);
-        f.idx = j << (8 - s);
+        if (R22) {
+            const unsigned sd = logn2 - 1 - s;  // the transposed DIF stage
+            f.idx = ((sd & 1) == 0 && sd + 1 < logn2 && 2 * j >= (1u << s)) ? 128u : 0u;
+        } else {
+            f.idx = j << (8 - s);
+        }
     } else if (u + 1 == logn2) {  // middle: positions j, j + n2/2
         f.p0 = b;
         f.p1 = b + (1u << (logn2 - 1));
-        f.idx = b << (9 - logn2);
+        f.idx = R22 ? (4 * b >= (1u << logn2) ? 128u : 0u) : b << (9 - logn2);
     } else {  // FFT DIF stage s = u - (L-1): span 2^(L-s-1)
         const unsigned s = u + 1 - logn2, lh = logn2 - s - 1, j = b & ((1u << lh) - 1);
         f.p0 = ((b >> lh) << (lh + 1)) + j;
         f.p1 = f.p0 + (1u << lh);
-        f.idx = j << (8 - lh);
+        if (R22) f.idx = ((s & 1) == 0 && s + 1 < logn2 && 2 * j >= (1u << lh)) ? 128u : 0u;
+        else f.idx = j << (8 - lh);
     }
     return f;
+}
+
+// The point-wise twiddles of odd DIF stage sd (span h = 2^(L-1-sd) >= 2) in the radix-2^2
+// schedule: in every block of 4h positions, quarter c (0..3) at offset j1 (0 < j1 < h) is
+// multiplied by w^(m_c j1 2^(sd-1)), m = (0, 2, 1, 3), in units of w of order 2^L (w512^(e)
+// with e scaled by 2^(9-L)).  Nontrivial twiddles are numbered k = 0 .. layer_count - 1.
+struct LayerTw {
+    unsigned q, e;  // position, exponent of w512
+};
+
+__host__ __device__ __forceinline__ unsigned layer_count(unsigned sd, unsigned logn2) {
+    const unsigned h = 1u << (logn2 - 1 - sd);
+    return h < 2 ? 0 : 3 * (h - 1) * ((1u << logn2) / (4 * h));
+}
+
+__device__ __forceinline__ LayerTw layer_twiddle(unsigned sd, unsigned logn2, unsigned k) {
+    const unsigned lh = logn2 - 1 - sd, h = 1u << lh, per = 3 * (h - 1);
+    const unsigned blk = k / per, r = k - blk * per;
+    const unsigned c = 1 + r / (h - 1), j1 = 1 + r % (h - 1);
+    LayerTw t;
+    t.q = (blk << (lh + 2)) + (c << lh) + j1;
+    t.e = ((c == 1 ? 2u : c == 2 ? 1u : 3u) * j1) << (sd - 1 + 9 - logn2);
+    return t;
 }
 
 // IFFT decimation in time (bit-reversed input): x = D[p1] w^-idx; D[p0], D[p1] = y + x, y - x.
@@ -514,7 +554,7 @@ __device__ __forceinline__ void run_butterfly(const Item &it, unsigned u, unsign
                                               const uint32_t *index) {
     const unsigned rows = it.rows, rg = (rows + 63) / 64;
     const unsigned row = min((local % rg) * 64 + threadIdx.x, rows - 1);
-    const Butterfly f = stage_positions(u, it.logn2, local / rg);
+    const Butterfly f = stage_positions<false>(u, it.logn2, local / rg);
     if (u + 1 < it.logn2) dit_butterfly<W>(it.D, rows, row, f, ops, index);
     else if (u + 1 == it.logn2) middle_butterfly<W>(it.D, rows, row, f, ops, index);
     else dif_butterfly<W>(it.D, rows, row, f, ops, index);
@@ -522,20 +562,41 @@ __device__ __forceinline__ void run_butterfly(const Item &it, unsigned u, unsign
 
 // One launch per stage: it.stage = s of this launch's kind (DIT/DIF), 0 for the middle.
 // Each kernel inlines only its own butterfly (the chain kernel below needs all three).
+// Lanes past the item's last row repeat the last row's work (identical loads and stores
+// within one wave) rather than leaving the wave: a ladder run by a wave with one active
+// lane measured 2.3x slower than the same ladder with all 64 lanes active.
 #define FK20_STAGE_KERNEL(NAME, BODY, USTAGE)                                                       \
     template <int W>                                                                             \
     __global__ void __launch_bounds__(64, 2) NAME(Launch L, const uint32_t *ops, const uint32_t *index) { \
         unsigned local;                                                                          \
         const Item &it = L.item[find_item(L, blockIdx.x, local)];                                \
-        const unsigned rg = (it.rows + 63) / 64, row = (local % rg) * 64 + threadIdx.x;          \
-        if (row >= it.rows) return;                                                              \
-        const Butterfly f = stage_positions((USTAGE), it.logn2, local / rg);                     \
+        const unsigned rg = (it.rows + 63) / 64;                                                 \
+        const unsigned row = min((local % rg) * 64 + threadIdx.x, it.rows - 1);                  \
+        const Butterfly f = stage_positions<true>((USTAGE), it.logn2, local / rg);               \
         BODY<W>(it.D, it.rows, row, f, ops, index);                                              \
     }
 FK20_STAGE_KERNEL(k_dit_multi, dit_butterfly, (unsigned)it.stage)
 FK20_STAGE_KERNEL(k_middle_multi, middle_butterfly, it.logn2 - 1)
 FK20_STAGE_KERNEL(k_dif_multi, dif_butterfly, it.logn2 - 1 + (unsigned)it.stage)
 #undef FK20_STAGE_KERNEL
+
+// The point-wise twiddle layer of odd DIF stage it.stage (radix-2^2 schedule): after that
+// stage's butterflies in the FFT (dir = TW_FWD), before the butterflies of the transposed DIT
+// stage in the IFFT (dir = TW_INV).  One wave per (nontrivial position, 64-row group).
+template <int W>
+__global__ void __launch_bounds__(64, 2) k_twiddle_multi(Launch L, const uint32_t *ops, const uint32_t *index,
+                                                        unsigned dir) {
+    unsigned local;
+    const Item &it = L.item[find_item(L, blockIdx.x, local)];
+    const unsigned rg = (it.rows + 63) / 64;
+    const unsigned row = min((local % rg) * 64 + threadIdx.x, it.rows - 1);
+    const LayerTw t = layer_twiddle((unsigned)it.stage, it.logn2, local / rg);
+    uint32_t *p = it.D + ((size_t)t.q * it.rows + row) * 36;
+    g1p x;
+    pt_load(x, p);
+    tw_mul<W>(x, ops + index[dir * 512 + t.e]);
+    pt_store(p, x);
+}
 
 // ---------------------------------------------------------------- persistent transform chain
 //
@@ -579,7 +640,7 @@ __global__ void __launch_bounds__(64, 2) k_chain(Chain C, const uint32_t *ops, c
         const Item &it = C.L.item[k];
         const unsigned u = (unsigned)C.ustage[t][k];
         const unsigned rg = (it.rows + 63) / 64, g = local % rg;
-        const Butterfly f = stage_positions(u, it.logn2, local / rg);
+        const Butterfly f = stage_positions<false>(u, it.logn2, local / rg);
         uint32_t *fl = it.flags + (size_t)g * (1u << it.logn2);
         if (u) {
             const uint64_t t0 = wall_clock64();
@@ -645,8 +706,7 @@ template <int W>
 __global__ void __launch_bounds__(64, 2) k_plain_load(uint32_t *D, const uint32_t *in, unsigned rows, unsigned logn,
                                                      unsigned brev, const uint32_t *scale_ops) {
     const unsigned rg = (rows + 63) / 64, j = blockIdx.x / rg;
-    const unsigned row = (blockIdx.x % rg) * 64 + threadIdx.x;
-    if (row >= rows) return;
+    const unsigned row = min((blockIdx.x % rg) * 64 + threadIdx.x, rows - 1);  // spare lanes repeat the last row
     uint32_t w[36];
     ld_words(w, in + ((size_t)row * (1u << logn) + j) * G1P_WORDS, 9);
     g1p p;
@@ -765,8 +825,23 @@ hipError_t launch_batch(const Launch &base, const DevTables *T, const uint32_t *
         if (acc) hipLaunchKernelGGL(k_chain<TW_WNAF>, dim3(std::min(acc, CHAIN_WAVES)), dim3(64), 0, s, C, ops, index);
         hook(LK_CHAIN);
     } else {
-        // one launch per stage; IFFT DIT stages right-aligned to the middle launch
+        // one launch per stage; IFFT DIT stages right-aligned to the middle launch.  A DIT stage
+        // whose transposed DIF stage is odd is preceded by that stage's twiddle layer.
         for (unsigned t = 0; t + 1 < lmax; t++) {
+            for (unsigned k = 0; k < L.nitems; k++) {
+                const Item &it = L.item[k];
+                const int st = (int)t - (int)(lmax - it.logn2);
+                const unsigned sd = it.logn2 - 1 - (unsigned)st;
+                const bool layer = st >= 0 && (sd & 1) && layer_count(sd, it.logn2);
+                L.item[k].stage = layer ? (int)sd : -1;
+                per[k] = layer ? layer_count(sd, it.logn2) * ((it.rows + 63) / 64) : 0;
+            }
+            total = set_blocks(L, per);
+            if (total) {
+                hipLaunchKernelGGL(k_twiddle_multi<TW_WNAF>, dim3(total), dim3(64), 0, s, L, ops, index,
+                                   (unsigned)TW_INV);
+                hook(LK_TWIDDLE);
+            }
             for (unsigned k = 0; k < L.nitems; k++) {
                 const int st = (int)t - (int)(lmax - L.item[k].logn2);
                 L.item[k].stage = st;
@@ -793,6 +868,19 @@ hipError_t launch_batch(const Launch &base, const DevTables *T, const uint32_t *
             total = set_blocks(L, per);
             if (total) hipLaunchKernelGGL(k_dif_multi<TW_WNAF>, dim3(total), dim3(64), 0, s, L, ops, index);
             hook(LK_DIF);
+            // odd DIF stages: their twiddle layer
+            for (unsigned k = 0; k < L.nitems; k++) {
+                const Item &it = L.item[k];
+                const bool layer = t < it.logn2 && (t & 1) && layer_count(t, it.logn2);
+                L.item[k].stage = layer ? (int)t : -1;
+                per[k] = layer ? layer_count(t, it.logn2) * ((it.rows + 63) / 64) : 0;
+            }
+            total = set_blocks(L, per);
+            if (total) {
+                hipLaunchKernelGGL(k_twiddle_multi<TW_WNAF>, dim3(total), dim3(64), 0, s, L, ops, index,
+                                   (unsigned)TW_FWD);
+                hook(LK_TWIDDLE);
+            }
         }
     }
     // output
@@ -826,11 +914,24 @@ hipError_t launch_plain(PlainKind kind, uint32_t *out, const uint32_t *in, const
     const unsigned rg = (rows + 63) / 64, n = 1u << logn;
     unsigned per[MAX_ITEMS] = {};
     const uint32_t *scale = P.ops + P.scale_off[logn];
+    // radix-2^2 twiddle layers (see stage_positions): DIF stage t's after its butterflies,
+    // the transposed DIF stage's before DIT stage t
+    auto layer = [&](unsigned sd, unsigned dir) {
+        const unsigned cnt = (sd & 1) ? layer_count(sd, logn) : 0;
+        if (!cnt) return;
+        it.stage = (int)sd;
+        per[0] = cnt * rg;
+        hipLaunchKernelGGL(k_twiddle_multi<TW_WNAF>, dim3(set_blocks(L, per)), dim3(64), 0, s, L, P.ops, P.index,
+                           dir);
+    };
     auto stage = [&](void (*kernel)(Launch, const uint32_t *, const uint32_t *), int st) {
+        const bool dit = kernel == k_dit_multi<TW_WNAF>, dif = kernel == k_dif_multi<TW_WNAF>;
+        if (dit) layer(logn - 1 - (unsigned)st, (unsigned)TW_INV);
         it.stage = st;
         per[0] = n / 2 * rg;
         const unsigned total = set_blocks(L, per);
         hipLaunchKernelGGL(kernel, dim3(total), dim3(64), 0, s, L, P.ops, P.index);
+        if (dif) layer((unsigned)st, (unsigned)TW_FWD);
     };
     switch (kind) {
     case PLAIN_G1_FFT:

@@ -190,4 +190,4 @@ def twiddle_selfcheck():
 OUTPUT_FORMATS = {"g1p": 0, "g1p_da": 1, "compressed": 2, "compressed_da": 3}
 
 LAUNCH_KINDS = {0: "k_toeplitz_fft_T", 1: "k_msm_comb", 2: "k_dit_stage", 3: "k_middle", 4: "k_dif_stage",
-                5: "k_fft_output", 6: "k_transform_chain"}
+                5: "k_fft_output", 6: "k_transform_chain", 7: "k_twiddle_layer"}

@@ -60,20 +60,33 @@ def row_hbm_bytes(N):
 def launch_mul32(N, kind, rows):
     """Canonical work of one launch of the batched path over `rows` rows.
 
-    k_msm_comb: the MSM; k_dit_stage / k_dif_stage: one radix-2 stage of (n2/2) butterflies;
-    k_middle: the last IFFT and first FFT stages (two stages); k_transform_chain: all of
-    them (2 log2(n2) stages).  The IFFT's n2 scaling
-    multiplications (folded into the setup tables here) are not attributed to any launch.
+    k_msm_comb: the MSM; k_transform_chain: all 2 log2(n2) radix-2 stages of (n2/2)
+    butterflies (BF each).  The per-stage launches run the radix-2^2 schedule, which moves
+    most twiddles out of the butterflies: their canonical share is split as
+    (n2/2) * 2 add per butterfly launch (k_dit_stage / k_dif_stage; k_middle: the two
+    stages it fuses) and one canonical scalar multiplication SM per twiddle a launch applies
+    (k_twiddle_layer: the mean over a pass's layer launches; the butterfly launches' w^+-128
+    twiddles likewise).  The IFFT's n2 scaling multiplications (folded into the setup tables
+    here) are not attributed to any launch.
     """
     n2 = N // 8
+    lg = int(math.log2(n2))
+    if kind in ("k_dit_stage", "k_dif_stage", "k_middle"):
+        if kind == "k_middle":
+            tw = 2 * sum(1 for b in range(n2 // 2) if r22_butterfly_idx(lg - 1, lg, b))
+            return rows * (2 * (n2 // 2) * 2 * ADD + tw * SM) * FP_MUL32
+        us = range(lg - 1) if kind == "k_dit_stage" else range(lg, 2 * lg - 1)
+        tw = sum(1 for u in us for b in range(n2 // 2) if r22_butterfly_idx(u, lg, b)) / len(us)
+        return rows * ((n2 // 2) * 2 * ADD + tw * SM) * FP_MUL32
+    if kind == "k_twiddle_layer":
+        layers = r22_layers(N)
+        tw = sum(len(r22_layer(sd, lg)) for sd, _ in layers) / len(layers)
+        return rows * tw * SM * FP_MUL32
     stage = (n2 // 2) * BF * FP_MUL32
     return rows * {
         "k_msm_comb": 16 * n2 * (SM + ADD) * FP_MUL32,
-        "k_dit_stage": stage,
-        "k_dif_stage": stage,
-        "k_middle": 2 * stage,
-        "k_transform_chain": 2 * int(math.log2(n2)) * stage,
-        "k_toeplitz_fft_T": 16 * (n2 // 2) * int(math.log2(n2)) * FR_MUL32,
+        "k_transform_chain": 2 * lg * stage,
+        "k_toeplitz_fft_T": 16 * (n2 // 2) * lg * FR_MUL32,
         "k_fft_output": 0,
     }[kind]
 
@@ -173,34 +186,88 @@ def twiddle_fp_muls(e_idx, inverse=False):
     return table + dbl * DBL + adds * ADD + phi
 
 
+def r22_butterfly_idx(u, lg, b):
+    """Twiddle exponent (units of w512) of butterfly b of local stage u in the radix-2^2
+    schedule of the per-stage launches (fast_kernels.hip stage_positions<true>): only the
+    even DIF stages (and the transposed DIT stages) keep a w^(n/4) = w512^128 part."""
+    n2 = 1 << lg
+    if u + 1 < lg:  # DIT stage s = u
+        s = u
+        j = b & ((1 << s) - 1)
+        sd = lg - 1 - s
+        return 128 if (sd % 2 == 0 and sd + 1 < lg and 2 * j >= (1 << s)) else 0
+    if u + 1 == lg:  # middle
+        return 128 if 4 * b >= n2 else 0
+    s = u + 1 - lg
+    lh = lg - s - 1
+    j = b & ((1 << lh) - 1)
+    return 128 if (s % 2 == 0 and s + 1 < lg and 2 * j >= (1 << lh)) else 0
+
+
+def r22_layer(sd, lg):
+    """(position, exponent of w512) of every nontrivial twiddle of the point-wise layer of odd
+    DIF stage sd (fast_kernels.hip layer_twiddle)."""
+    h = 1 << (lg - 1 - sd)
+    if h < 2:
+        return []
+    n2 = 1 << lg
+    out = []
+    for k in range(3 * (h - 1) * (n2 // (4 * h))):
+        blk, r = divmod(k, 3 * (h - 1))
+        c, j1 = 1 + r // (h - 1), 1 + r % (h - 1)
+        out.append((blk * 4 * h + c * h + j1, ({1: 2, 2: 1, 3: 3}[c] * j1) << (sd - 1 + 9 - lg)))
+    return out
+
+
+def r22_layers(N):
+    """The twiddle-layer launches of one pass, as (sd, inverse): before DIT stages (inverse)
+    and after DIF stages (forward)."""
+    lg = int(math.log2(N // 8))
+    dit = [(lg - 1 - u, True) for u in range(lg - 1) if (lg - 1 - u) % 2 == 1 and r22_layer(lg - 1 - u, lg)]
+    dif = [(s, False) for s in range(1, lg) if s % 2 == 1 and r22_layer(s, lg)]
+    return dit + dif
+
+
 def executed_launch_fp_muls(N, kind, stage=None, window=8):
-    """Executed Fp muls of one launch of the batched path, per row."""
+    """Executed Fp muls of one launch of the batched path, per row.  k_dit_stage /
+    k_dif_stage / k_middle / k_twiddle_layer (stage = (sd, inverse)) follow the radix-2^2
+    schedule of the per-stage launches; k_transform_chain the radix-2 schedule of the
+    persistent chain."""
     n2 = N // 8
     lg = int(math.log2(n2))
     if kind == "k_msm_comb":
         # ceil(256/w) signed windows per term; zero digits (probability ~2^-w) ignored
         return 16 * n2 * ((256 + window - 1) // window) * MIXED_ADD
     if kind in ("k_dit_stage", "k_dif_stage"):
+        u = stage if kind == "k_dit_stage" else lg - 1 + stage
         total = 0
         for b in range(n2 // 2):
-            if kind == "k_dit_stage":
-                j = b & ((1 << stage) - 1)
-                idx = j << (8 - stage)
-            else:
-                lh = lg - stage - 1
-                j = b & ((1 << lh) - 1)
-                idx = j << (8 - lh)
+            idx = r22_butterfly_idx(u, lg, b)
             total += 2 * ADD + (twiddle_fp_muls(idx, inverse=(kind == "k_dit_stage")) if idx else 0)
         return total
-    if kind == "k_transform_chain":
-        return (sum(executed_launch_fp_muls(N, "k_dit_stage", s) for s in range(lg - 1))
-                + executed_launch_fp_muls(N, "k_middle")
-                + sum(executed_launch_fp_muls(N, "k_dif_stage", s) for s in range(1, lg)))
     if kind == "k_middle":
         total = 0
-        for j in range(n2 // 2):
+        for b in range(n2 // 2):
+            idx = r22_butterfly_idx(lg - 1, lg, b)
+            total += ADD + (twiddle_fp_muls(idx, True) + twiddle_fp_muls(idx, False) if idx else 0)
+        return total
+    if kind == "k_twiddle_layer":
+        sd, inverse = stage
+        return sum(twiddle_fp_muls(e, inverse) for _, e in r22_layer(sd, lg))
+    if kind == "k_transform_chain":
+        total = 0
+        for s in range(lg - 1):  # radix-2 DIT stages
+            for b in range(n2 // 2):
+                idx = (b & ((1 << s) - 1)) << (8 - s)
+                total += 2 * ADD + (twiddle_fp_muls(idx, True) if idx else 0)
+        for j in range(n2 // 2):  # middle
             idx = j << (9 - lg)
             total += ADD + (twiddle_fp_muls(idx, True) + twiddle_fp_muls(idx, False) if idx else 0)
+        for s in range(1, lg):  # radix-2 DIF stages
+            lh = lg - s - 1
+            for b in range(n2 // 2):
+                idx = (b & ((1 << lh) - 1)) << (8 - lh)
+                total += 2 * ADD + (twiddle_fp_muls(idx) if idx else 0)
         return total
     raise KeyError(kind)
 
@@ -211,4 +278,5 @@ def executed_row_fp_muls(N, window=8):
     t += sum(executed_launch_fp_muls(N, "k_dit_stage", s) for s in range(lg - 1))
     t += executed_launch_fp_muls(N, "k_middle")
     t += sum(executed_launch_fp_muls(N, "k_dif_stage", s) for s in range(1, lg))
+    t += sum(executed_launch_fp_muls(N, "k_twiddle_layer", st) for st in r22_layers(N))
     return t

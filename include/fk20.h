@@ -234,7 +234,8 @@ int fk20_host_free(void *ptr);
 /* Per-launch timing of fk20_prove_rows (direct launches, graph replay off while enabled):
  * an event is recorded after every kernel; fk20_ctx_launch_times waits for the last call's
  * events and returns per-launch milliseconds and kinds (0 Toeplitz+NTT, 1 MSM, 2 IFFT stage,
- * 3 middle stage, 4 FFT stage, 5 output). */
+ * 3 middle stage, 4 FFT stage, 5 output, 6 persistent transform, 7 twiddle layer of the
+ * radix-2^2 transform schedule). */
 int fk20_ctx_set_launch_timing(fk20_ctx_t *ctx, int enable);
 int fk20_ctx_launch_times(fk20_ctx_t *ctx, float *ms, int *kinds, int max, int *count);
 /* Host-only check of the twiddle programs (GLV split + wNAF of every w^+-i, i < 512,

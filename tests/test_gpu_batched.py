@@ -147,9 +147,10 @@ def test_uneven_small_batch_one_pass(prover):
 
 
 def test_chain_equals_per_stage_launches(prover):
-    """The persistent transform chain and one launch per stage (default) give bit-identical
-    projective outputs, on an uneven batch with partial 64-row groups; the chain also matches
-    the closed form."""
+    """The persistent transform chain (radix-2 schedule) and one launch per stage (default,
+    radix-2^2 schedule) give the same points (canonical affine; the projective representatives
+    differ with the schedule), on an uneven batch with partial 64-row groups; the chain also
+    matches the closed form."""
     groups, taus = [], []
     for N, rows, seed in ((1024, 70, 401), (4096, 66, 402), (2048, 1, 403)):
         tau = S.random_secret(seed)
@@ -163,7 +164,7 @@ def test_chain_equals_per_stage_launches(prover):
     finally:
         prover.set_chain(False)
     for (sid, poly), tau, a, b in zip(groups, taus, chained, staged):
-        assert np.array_equal(a, b)
+        assert np.array_equal(aff(a), aff(b))
         r = poly.shape[0] - 1
         assert np.array_equal(aff(a[r]), aff(shadow_points(poly[r], tau)))
     for sid, _ in groups:
