@@ -32,14 +32,32 @@ def _kernel(name):
     return name[5:] if name.startswith("void ") else name
 
 
+def _grid(r):
+    """Work-items of a dispatch, from a counter-collection row (Grid_Size) or a kernel-trace
+    row (Grid_Size_X/Y/Z)."""
+    if "Grid_Size" in r:
+        return int(r["Grid_Size"])
+    return int(r["Grid_Size_X"]) * int(r["Grid_Size_Y"]) * int(r["Grid_Size_Z"])
+
+
+def _workload(rows):
+    """The dispatches of the benchmarked workload: per kernel, those with at least a quarter of
+    the kernel's largest grid.  This drops the setup's small launches of the same kernels (the
+    16-row G1 FFT of fk20_setup2xext_fft) while keeping every launch of a pass (the twiddle
+    layers of one pass differ in size by at most 2x)."""
+    big = collections.defaultdict(int)
+    for r in rows:
+        big[_kernel(r["Kernel_Name"])] = max(big[_kernel(r["Kernel_Name"])], _grid(r))
+    return [r for r in rows if 4 * _grid(r) >= big[_kernel(r["Kernel_Name"])]]
+
+
 def read_pass(path, counter):
-    """{kernel: [KiB per dispatch]} for one counter pass."""
+    """{kernel: [KiB per dispatch]} for one counter pass (workload dispatches only)."""
     out = collections.defaultdict(list)
     with open(path) as f:
-        for r in csv.DictReader(f):
-            if r["Counter_Name"] != counter:
-                continue
-            out[_kernel(r["Kernel_Name"])].append(float(r["Counter_Value"]))
+        rows = [r for r in csv.DictReader(f) if r["Counter_Name"] == counter]
+    for r in _workload(rows):
+        out[_kernel(r["Kernel_Name"])].append(float(r["Counter_Value"]))
     return out
 
 
@@ -75,11 +93,11 @@ def valu_summary(counter_csv, trace_csv):
     workload), lane-instruction rate and VALU-busy fraction."""
     cnt = collections.defaultdict(lambda: collections.defaultdict(list))
     with open(counter_csv) as f:
-        for r in csv.DictReader(f):
+        for r in _workload(list(csv.DictReader(f))):
             cnt[_kernel(r["Kernel_Name"])][r["Counter_Name"]].append(float(r["Counter_Value"]))
     dur = collections.defaultdict(list)
     with open(trace_csv) as f:
-        for r in csv.DictReader(f):
+        for r in _workload(list(csv.DictReader(f))):
             dur[_kernel(r["Kernel_Name"])].append((int(r["End_Timestamp"]) - int(r["Start_Timestamp"])) * 1e-9)
     out = {}
     for k, c in cnt.items():
@@ -151,7 +169,9 @@ def main():
                  "Workload: %s.  Counters are summed over all SEs; GRBM_GUI_ACTIVE over the 8 XCDs "
                  "(cycles = value / 8); SQ_ACTIVE_INST_VALU is in quad-cycles, so VALU busy = "
                  "4 x SQ_ACTIVE_INST_VALU / (1024 SIMDs x cycles).  Durations come from the kernel "
-                 "trace of the same workload." % ", ".join("%s=%d" % kv for kv in sorted(cfg.items())), "",
+                 "trace of the same workload.  Per kernel, only dispatches with at least a quarter of "
+                 "its largest grid are counted (the setup's 16-row launches of the transform kernels "
+                 "are left out)." % ", ".join("%s=%d" % kv for kv in sorted(cfg.items())), "",
                  "| kernel | dispatches | VALU wave-instr / launch | INT64 share | INT32 share | avg ms | "
                  "T lane-instr/s | clock GHz | VALU busy |", "|---|---:|---:|---:|---:|---:|---:|---:|---:|"]
         for name, e in sorted(kern.items(), key=lambda kv: -kv[1]["valu_wave_instr"] * kv[1]["dispatches"]):
@@ -177,7 +197,9 @@ def main():
     lines = ["# HBM traffic per launch (rocprofv3 FETCH_SIZE / WRITE_SIZE, calibrated)", "",
              "Workload: %s." % ", ".join("%s=%d" % kv for kv in sorted(s["config"].items())), "",
              "Calibration: a %d MiB device-to-device copy read FETCH_SIZE = %.1f KiB and WRITE_SIZE = "
-             "%.1f KiB, so FETCH_SIZE is scaled by %.4f and WRITE_SIZE by %.4f." %
+             "%.1f KiB, so FETCH_SIZE is scaled by %.4f and WRITE_SIZE by %.4f.  Per kernel, only "
+             "dispatches with at least a quarter of its largest grid are counted (the setup's 16-row "
+             "launches of the transform kernels are left out)." %
              (args.copy_mb, s["copy_fetch_kib"], s["copy_write_kib"], s["fetch_scale"], s["write_scale"]), "",
              "| kernel | dispatches | fetch MB / launch | write MB / launch |", "|---|---:|---:|---:|"]
     for name, v in s["kernels"].items():
