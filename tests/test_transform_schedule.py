@@ -118,3 +118,27 @@ def test_radix22_twiddle_counts():
     radix2 = sum(1 for s in range(1, L) for b in range(256)
                  if (b & ((1 << s) - 1)) << (8 - s) not in (0, 128))
     assert radix2 == 1538
+
+
+def quarter_order(b, L, lh):
+    """Mirror of fast_kernels.hip quarter_order: launch index b of a w^+-128 stage (span
+    2^lh) -> (block, offset j), the untwiddled half of the launch first."""
+    c, r = b >> (L - 2), b & ((1 << (L - 2)) - 1)
+    return r >> (lh - 1), (r & ((1 << (lh - 1)) - 1)) + (c << (lh - 1))
+
+
+@pytest.mark.parametrize("L", range(3, 10))
+def test_quarter_order_is_a_bijection_in_bounds(L):
+    """The kernels renumber the butterflies of every w^+-128 stage: each launch index maps to a
+    distinct butterfly of the stage, both positions stay below n2, and the second half of the
+    launch is exactly the twiddled half (offset j >= span / 2)."""
+    n2 = 1 << L
+    for lh in range(1, L):
+        seen = set()
+        for b in range(n2 // 2):
+            blk, j = quarter_order(b, L, lh)
+            p0 = (blk << (lh + 1)) + j
+            assert 0 <= j < (1 << lh) and 0 <= p0 and p0 + (1 << lh) < n2
+            assert (2 * j >= (1 << lh)) == (b >= n2 // 4)
+            seen.add((blk, j))
+        assert len(seen) == n2 // 2
