@@ -175,6 +175,7 @@ const char *fk20_status_string(int status) {
     case FK20_ENOMEM: return "out of device memory";
     case FK20_EHIP: return "HIP runtime error";
     case FK20_ENODEV: return "no gfx950 device";
+    case FK20_ETIMEDOUT: return "a bounded device-side wait expired";
     default: return "unknown status";
     }
 }

@@ -249,6 +249,15 @@ struct fk20_ctx {
     size_t work_bytes = 0;
     bool use_graph = false;
     bool use_chain = false;  // transform stages as one persistent launch (k_chain)
+    uint64_t chain_wait = CHAIN_WAIT_TICKS;
+    // control blocks of the persistent launches of the last enqueued (or captured) pass: word 1
+    // is set by a wave whose bounded wait expired
+    std::vector<uint32_t *> chain_ctl;
+    // device-side order of the calls that use `work`: `done` is recorded on the stream of the
+    // last such call, and a call on another stream waits for it first
+    hipEvent_t done = nullptr;
+    hipStream_t done_stream = nullptr;
+    bool done_valid = false;
     int out_format = FK20_OUT_G1P;
     // fk20_prove_batch splits its rows into `nsub` halves at 64-row boundaries and runs each
     // half's launch sequence on its own stream, forked from and joined back to the caller's
@@ -323,6 +332,7 @@ int fk20_ctx_destroy(fk20_ctx_t *c) {
         if (c->join[k]) (void)hipEventDestroy(c->join[k]);
     }
     if (c->fork) (void)hipEventDestroy(c->fork);
+    if (c->done) (void)hipEventDestroy(c->done);
     delete c;
     return FK20_OK;
 }
@@ -372,6 +382,18 @@ int fk20_ctx_set_chain(fk20_ctx_t *c, int enable) {
         c->gkey.clear();
     }
     c->use_chain = enable != 0;
+    return FK20_OK;
+}
+
+int fk20_ctx_set_chain_wait(fk20_ctx_t *c, uint64_t ticks) {
+    if (!c) return FK20_EINVAL;
+    std::lock_guard<std::mutex> lk(c->mu);
+    if (c->chain_wait != ticks && c->gexec) {
+        (void)hipGraphExecDestroy(c->gexec);  // the bound is a captured launch argument
+        c->gexec = nullptr;
+        c->gkey.clear();
+    }
+    c->chain_wait = ticks;
     return FK20_OK;
 }
 
@@ -452,6 +474,21 @@ int fk20_commit_setup_load(fk20_ctx_t *c, const fk20_g1p_t *srs, unsigned N, uns
     return FK20_OK;
 }
 
+// Orders a call that uses the context workspace on stream s after the previous one.
+static hipError_t order_before(fk20_ctx *c, hipStream_t s) {
+    if (!c->done_valid || c->done_stream == s) return hipSuccess;
+    return hipStreamWaitEvent(s, c->done, 0);
+}
+
+static hipError_t order_after(fk20_ctx *c, hipStream_t s) {
+    hipError_t e = hipSuccess;
+    if (!c->done) e = hipEventCreateWithFlags(&c->done, hipEventDisableTiming);
+    if (e == hipSuccess) e = hipEventRecord(c->done, s);
+    c->done_valid = e == hipSuccess;
+    c->done_stream = s;
+    return e;
+}
+
 size_t fk20_commit_setup_table_bytes(unsigned N, unsigned window) {
     if (!valid_N(N) || !valid_win(window)) return 0;
     return (size_t)N * nwin_of(window) * ((size_t)1 << (window - 1)) * 96;
@@ -472,12 +509,14 @@ int fk20_commit_rows(fk20_ctx_t *c, int setup_id, fk20_g1p_t *commitments, const
     if (!rows) return FK20_OK;
     if (!commitments || !polynomial) return FK20_EINVAL;
     const Setup &st = c->setups[setup_id];
+    hipStream_t s = reinterpret_cast<hipStream_t>(stream);
     hipError_t e = grow_work(c, commit_workspace_bytes(rows, st.N));
-    if (e != hipSuccess) return status_of(e);
-    return status_of(launch_commit(reinterpret_cast<uint32_t *>(commitments),
-                                   reinterpret_cast<const uint32_t *>(polynomial), st.table, st.inf,
-                                   reinterpret_cast<uint32_t *>(c->work), rows, st.N, st.win,
-                                   reinterpret_cast<hipStream_t>(stream)));
+    if (e == hipSuccess) e = order_before(c, s);
+    if (e == hipSuccess)
+        e = launch_commit(reinterpret_cast<uint32_t *>(commitments), reinterpret_cast<const uint32_t *>(polynomial),
+                          st.table, st.inf, reinterpret_cast<uint32_t *>(c->work), rows, st.N, st.win, s);
+    if (e == hipSuccess) e = order_after(c, s);
+    return status_of(e);
 }
 
 int fk20_setup_unload(fk20_ctx_t *c, int id) {
@@ -539,7 +578,7 @@ static bool same_items(const std::vector<fk20_batch_item_t> &a, const fk20_batch
 // Carves each item's workspace (tcT, D, chain flags, control block) from w and launches the
 // batch on s.
 static hipError_t enqueue_launch(fk20_ctx *c, const fk20_batch_item_t *items, unsigned n, hipStream_t s,
-                                 LaunchHook hook, char *w) {
+                                 LaunchHook hook, char *w, bool chain) {
     Launch L;
     memset(&L, 0, sizeof L);
     L.nitems = n;
@@ -567,16 +606,20 @@ static hipError_t enqueue_launch(fk20_ctx *c, const fk20_batch_item_t *items, un
         it.out_format = (uint32_t)c->out_format;
         it.stage = -1;
     }
-    return launch_batch(L, c->T, c->prog.ops, c->prog.index, s, hook, c->use_chain ? ctl : nullptr);
+    if (chain) c->chain_ctl.push_back(ctl);
+    return launch_batch(L, c->T, c->prog.ops, c->prog.index, s, hook, chain ? ctl : nullptr, c->chain_wait);
 }
 
 static hipError_t enqueue_items(fk20_ctx *c, const fk20_batch_item_t *items, unsigned n, hipStream_t s,
                                 LaunchHook hook, void *workspace = nullptr) {
+    // the persistent form only on the context's own workspace (fk20_prove_batch reads its
+    // error words back; the serving loop's fk20_prove_batch_ws does not wait for its batches)
+    const bool chain = c->use_chain && !workspace;
     char *w = reinterpret_cast<char *>(workspace ? workspace : c->work);
     unsigned groups = 0;
     for (unsigned k = 0; k < n; k++) groups += (items[k].rows + 63) / 64;
     // the context path with at least two 64-row groups, launches not timed one by one
-    if (workspace || c->nsub < 2 || groups < 2 || hook.fn) return enqueue_launch(c, items, n, s, hook, w);
+    if (workspace || c->nsub < 2 || groups < 2 || hook.fn) return enqueue_launch(c, items, n, s, hook, w, chain);
     // halves: each item's rows split at a 64-row boundary (items of one group go to the first)
     fk20_batch_item_t half[2][MAX_ITEMS];
     unsigned nh[2] = {0, 0};
@@ -592,7 +635,7 @@ static hipError_t enqueue_items(fk20_ctx *c, const fk20_batch_item_t *items, uns
         half[0][nh[0]++] = a;
         if (b.rows) half[1][nh[1]++] = b;
     }
-    if (!nh[1]) return enqueue_launch(c, items, n, s, hook, w);
+    if (!nh[1]) return enqueue_launch(c, items, n, s, hook, w, chain);
     // the halves' workspaces: a 64-row split keeps the flag words, and each half carves one
     // 256-byte control block of the two the whole item reserves, so both fit in the whole's size
     size_t off = 0;  // bytes enqueue_launch carves for the first half (256 below the reserved size per item)
@@ -610,8 +653,8 @@ static hipError_t enqueue_items(fk20_ctx *c, const fk20_batch_item_t *items, uns
     if (e == hipSuccess && !c->fork) e = hipEventCreateWithFlags(&c->fork, hipEventDisableTiming);
     if (e == hipSuccess) e = hipEventRecord(c->fork, s);
     for (int h = 0; h < 2 && e == hipSuccess; h++) e = hipStreamWaitEvent(c->sub[h], c->fork, 0);
-    if (e == hipSuccess) e = enqueue_launch(c, half[0], nh[0], c->sub[0], hook, w);
-    if (e == hipSuccess) e = enqueue_launch(c, half[1], nh[1], c->sub[1], hook, w + off);
+    if (e == hipSuccess) e = enqueue_launch(c, half[0], nh[0], c->sub[0], hook, w, chain);
+    if (e == hipSuccess) e = enqueue_launch(c, half[1], nh[1], c->sub[1], hook, w + off, chain);
     for (int h = 0; h < 2 && e == hipSuccess; h++) {
         e = hipEventRecord(c->join[h], c->sub[h]);
         if (e == hipSuccess) e = hipStreamWaitEvent(s, c->join[h], 0);
@@ -637,35 +680,52 @@ int fk20_prove_batch(fk20_ctx_t *c, const fk20_batch_item_t *items, unsigned nit
     const unsigned n = (unsigned)v.size();
     hipStream_t s = reinterpret_cast<hipStream_t>(stream);
     hipError_t e = grow_work(c, items_workspace_bytes(c, v.data(), n));
+    if (e == hipSuccess) e = order_before(c, s);
     if (e != hipSuccess) return status_of(e);
     if (c->timing) {
         // direct launches with an event after each one (read back by fk20_ctx_launch_times)
         c->tstream = s;
         c->nmarks = 0;
+        c->chain_ctl.clear();
         e = hipEventRecord(c->events[0], s);
-        if (e != hipSuccess) return status_of(e);
-        return status_of(enqueue_items(c, v.data(), n, s, LaunchHook{mark_launch, c}));
-    }
-    if (!c->use_graph || s == nullptr) return status_of(enqueue_items(c, v.data(), n, s, LaunchHook{nullptr, nullptr}));
-    // hipGraph: capture the launch sequence once per (items, stream), replay after
-    if (!c->gexec || c->gstream != s || !same_items(c->gkey, v.data(), n)) {
-        if (c->gexec) {
-            (void)hipGraphExecDestroy(c->gexec);
-            c->gexec = nullptr;
+        if (e == hipSuccess) e = enqueue_items(c, v.data(), n, s, LaunchHook{mark_launch, c});
+    } else if (!c->use_graph || s == nullptr) {
+        c->chain_ctl.clear();
+        e = enqueue_items(c, v.data(), n, s, LaunchHook{nullptr, nullptr});
+    } else {
+        // hipGraph: capture the launch sequence once per (items, stream), replay after
+        if (!c->gexec || c->gstream != s || !same_items(c->gkey, v.data(), n)) {
+            if (c->gexec) {
+                (void)hipGraphExecDestroy(c->gexec);
+                c->gexec = nullptr;
+            }
+            hipGraph_t g = nullptr;
+            c->chain_ctl.clear();
+            e = hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal);
+            if (e != hipSuccess) return status_of(e);
+            hipError_t le = enqueue_items(c, v.data(), n, s, LaunchHook{nullptr, nullptr});
+            e = hipStreamEndCapture(s, &g);
+            if (le != hipSuccess) e = le;
+            if (e == hipSuccess) e = hipGraphInstantiate(&c->gexec, g, nullptr, nullptr, 0);
+            if (g) (void)hipGraphDestroy(g);
+            if (e != hipSuccess) return status_of(e);
+            c->gkey = v;
+            c->gstream = s;
         }
-        hipGraph_t g = nullptr;
-        e = hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal);
-        if (e != hipSuccess) return status_of(e);
-        hipError_t le = enqueue_items(c, v.data(), n, s, LaunchHook{nullptr, nullptr});
-        e = hipStreamEndCapture(s, &g);
-        if (le != hipSuccess) e = le;
-        if (e == hipSuccess) e = hipGraphInstantiate(&c->gexec, g, nullptr, nullptr, 0);
-        if (g) (void)hipGraphDestroy(g);
-        if (e != hipSuccess) return status_of(e);
-        c->gkey = v;
-        c->gstream = s;
+        e = hipGraphLaunch(c->gexec, s);  // chain_ctl still holds the captured control blocks
     }
-    return status_of(hipGraphLaunch(c->gexec, s));
+    if (e == hipSuccess) e = order_after(c, s);
+    if (e != hipSuccess || !c->use_chain) return status_of(e);
+    // persistent form: wait for the pass and report an expired wait
+    e = hipStreamSynchronize(s);
+    uint32_t expired = 0;
+    for (uint32_t *ctl : c->chain_ctl) {
+        uint32_t word = 0;
+        if (e == hipSuccess) e = hipMemcpy(&word, ctl + 1, 4, hipMemcpyDeviceToHost);
+        expired |= word;
+    }
+    if (e != hipSuccess) return status_of(e);
+    return expired ? FK20_ETIMEDOUT : FK20_OK;
 }
 
 int fk20_prove_batch_ws(fk20_ctx_t *c, const fk20_batch_item_t *items, unsigned nitems, void *workspace,

@@ -82,6 +82,7 @@ static constexpr unsigned MAX_STAGES = 17;  // 2 * 9 - 1 for n2 = 512
 struct Chain {
     Launch L;
     uint32_t *ctl;  // [0] work counter, [1] error word (a bounded wait expired)
+    uint64_t wait_ticks;  // bound of each wait, in ticks of the constant-rate wall clock
     uint32_t total, nstages;
     uint32_t t_end[MAX_STAGES];
     uint32_t blk_end[MAX_STAGES][MAX_ITEMS];
@@ -105,9 +106,12 @@ struct LaunchHook {
 // The whole batched path for the items of `L`: Toeplitz + NTT, MSM, 2*Lmax - 1 transform
 // stages, output -- one launch per stage covering every item.
 // chain != nullptr: the transform stages run as one persistent k_chain launch whose work
-// counter lives at `chain` (two words, cleared by the launch sequence itself).
+// counter and error word live at chain[0..1] (cleared by the launch sequence itself); each
+// of its waits gives up after chain_wait ticks and then sets chain[1].
+static constexpr uint64_t CHAIN_WAIT_TICKS = 1ull << 27;  // ~1.3 s at the 100 MHz wall clock
 hipError_t launch_batch(const Launch &L, const DevTables *T, const uint32_t *ops, const uint32_t *index,
-                        hipStream_t s, LaunchHook hook, uint32_t *chain = nullptr);
+                        hipStream_t s, LaunchHook hook, uint32_t *chain = nullptr,
+                        uint64_t chain_wait = CHAIN_WAIT_TICKS);
 
 // The drop-in entry points on the same column-batched transform kernels, for `rows`
 // transforms of n = 2^logn points (n2 = n for the FK20 kinds), with D = rows * n * 144

@@ -239,9 +239,10 @@ __global__ void __launch_bounds__(256) k_toeplitz_fft_multi(Launch L, const DevT
 
 // ---------------------------------------------------------------- window-table MSM
 
-// acc += k * B for a 256-bit standard-form scalar k (8 words, consumed) by signed fixed
-// windows of WIN bits over B's table tb[win][d - 1] = d * 2^(WIN*win) * B (affine): one
-// mixed addition per non-zero digit, no doublings.
+// acc += k * B for a standard-form scalar k < r (8 words, consumed) by signed fixed windows
+// of WIN bits over B's table tb[win][d - 1] = d * 2^(WIN*win) * B (affine): one mixed
+// addition per non-zero digit, no doublings.  k < r < 2^255 keeps the last window's carry
+// inside the ceil(256/WIN) windows for every WIN in 8..13.
 __device__ __forceinline__ void window_term(g1p &acc, uint32_t *k, const uint32_t *tb, unsigned WIN) {
     const unsigned NWIN = (256 + WIN - 1) / WIN, H = 1u << (WIN - 1), MASK = (1u << WIN) - 1;
     uint32_t carry = 0;
@@ -344,9 +345,13 @@ __global__ void __launch_bounds__(64) k_commit_partial(const uint32_t *poly, con
 #pragma unroll 1
     for (unsigned i = i0; i < i1; i++) {
         if (inf[i]) continue;
-        uint32_t k[8];
-        ld_words(k, poly + ((size_t)row * N + i) * FR_WORDS, 2);
-        window_term(acc, k, table + (size_t)i * NWIN * H * G1A_WORDS, win_bits);
+        // the caller's coefficient may be any 256-bit representative: reduce it below r, so
+        // the top signed window's carry stays inside the ceil(256/w) windows (at w = 8 the
+        // windows end exactly at bit 256, and a value >= 2^255 would carry out of them)
+        fs k;
+        ld_words(k.l, poly + ((size_t)row * N + i) * FR_WORDS, 2);
+        fs_reduce_std(k);
+        window_term(acc, k.l, table + (size_t)i * NWIN * H * G1A_WORDS, win_bits);
     }
     pt_store(partial + ((size_t)c * rows + row) * 36, acc);
 }
@@ -628,11 +633,9 @@ __global__ void __launch_bounds__(64, 2) k_twiddle_multi(Launch L, const uint32_
 // (untwiddled) butterfly moves on to later stages at once instead of idling at a launch
 // boundary.  Items are taken in stage order, so every item an item depends on was taken
 // earlier by a running wave: the waits always end.  Each wait is bounded all the same, by
-// the constant-rate wall clock (CHAIN_WAIT_TICKS, about 1.3 s at the 100 MHz reference;
-// a legitimate wait is at most one pass, tens of ms); on expiry the wave records an error
-// in ctl[1] and proceeds.
-
-static constexpr uint64_t CHAIN_WAIT_TICKS = 1ull << 27;
+// the constant-rate wall clock (C.wait_ticks, by default about 1.3 s at the 100 MHz
+// reference; a legitimate wait is at most one pass, tens of ms); on expiry the wave records
+// an error in ctl[1] and proceeds, and the host turns that word into an error status.
 
 __device__ __forceinline__ unsigned flag_load(const uint32_t *f) {
     return __hip_atomic_load(f, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -668,7 +671,7 @@ __global__ void __launch_bounds__(64, 2) k_chain(Chain C, const uint32_t *ops, c
             bool expired = false;
 #pragma unroll 1
             while (flag_load(fl + f.p0) < u || flag_load(fl + f.p1) < u) {
-                if (wall_clock64() - t0 > CHAIN_WAIT_TICKS) {
+                if (wall_clock64() - t0 >= C.wait_ticks) {
                     expired = true;
                     break;
                 }
@@ -792,7 +795,7 @@ static unsigned set_blocks(Launch &L, const unsigned *per_item) {
 static constexpr unsigned CHAIN_WAVES = 2048;
 
 hipError_t launch_batch(const Launch &base, const DevTables *T, const uint32_t *ops, const uint32_t *index,
-                        hipStream_t s, LaunchHook hook, uint32_t *chain) {
+                        hipStream_t s, LaunchHook hook, uint32_t *chain, uint64_t chain_wait) {
     Launch L = base;
     unsigned per[MAX_ITEMS], lmax = 0;
     for (unsigned k = 0; k < L.nitems; k++) lmax = std::max(lmax, L.item[k].logn2);
@@ -826,6 +829,7 @@ hipError_t launch_batch(const Launch &base, const DevTables *T, const uint32_t *
         memset(&C, 0, sizeof C);
         C.L = L;
         C.ctl = chain;
+        C.wait_ticks = chain_wait;
         C.nstages = nt;
         unsigned acc = 0;
         for (unsigned t = 0; t < nt; t++) {

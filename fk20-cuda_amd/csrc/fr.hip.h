@@ -107,6 +107,13 @@ FK20_HD void fs_zero(fs &z) {
     for (int i = 0; i < 8; i++) z.l[i] = 0;
 }
 
+// any 256-bit standard-form value -> its canonical representative in [0, r): 2^256 < 3 r, so
+// at most two subtractions of r
+FK20_HD void fs_reduce_std(fs &z) {
+    fs_cond_sub(z, 0);
+    fs_cond_sub(z, 0);
+}
+
 // standard form (any 256-bit value) -> canonical Montgomery
 FK20_HD void fs_to_mont(fs &z, const fs &a) {
     fs r2;

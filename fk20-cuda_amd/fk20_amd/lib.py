@@ -14,7 +14,7 @@ LIBPATH = os.environ.get("FK20_LIBRARY", os.path.join(PKG_DIR, "libfk20.so"))
 
 FK20_OK = 0
 STATUS = {0: "ok", -1: "invalid argument", -2: "out of device memory", -3: "HIP runtime error",
-          -4: "no gfx950 device"}
+          -4: "no gfx950 device", -5: "a bounded device-side wait expired"}
 
 
 class Fk20Error(RuntimeError):
@@ -52,6 +52,7 @@ _SIGS = {
     "fk20_ctx_destroy": ([_P], _I),
     "fk20_ctx_set_graph": ([_P, _I], _I),
     "fk20_ctx_set_chain": ([_P, _I], _I),
+    "fk20_ctx_set_chain_wait": ([_P, ctypes.c_uint64], _I),
     "fk20_ctx_set_streams": ([_P, _U], _I),
     "fk20_ctx_set_output": ([_P, _I], _I),
     "fk20_ctx_get_output": ([_P, ctypes.POINTER(_I)], _I),

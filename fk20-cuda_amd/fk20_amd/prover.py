@@ -149,6 +149,10 @@ class Prover:
         """Transform stages as one persistent launch, or one launch per stage (default)."""
         call("fk20_ctx_set_chain", self.ctx, int(bool(enable)))
 
+    def set_chain_wait(self, ticks):
+        """Bound of each wait of the persistent transform launch, in wall-clock ticks."""
+        call("fk20_ctx_set_chain_wait", self.ctx, int(ticks))
+
     def set_graph(self, enable):
         call("fk20_ctx_set_graph", self.ctx, int(bool(enable)))
 

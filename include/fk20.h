@@ -42,7 +42,8 @@ enum {
     FK20_EINVAL = -1,   /* bad size, count or null pointer */
     FK20_ENOMEM = -2,   /* device allocation failed or budget exceeded */
     FK20_EHIP = -3,     /* HIP runtime error (fk20_last_hip_error gives the code) */
-    FK20_ENODEV = -4    /* no usable gfx950 device */
+    FK20_ENODEV = -4,   /* no usable gfx950 device */
+    FK20_ETIMEDOUT = -5 /* a bounded device-side wait expired; the outputs are not valid */
 };
 
 /* Last HIP error code seen by the library in this thread (0 if none). */
@@ -113,8 +114,12 @@ int fk20_setup2xext_fft(fk20_g1p_t *xext_fft, const fk20_g1p_t *setup, unsigned 
  * setup, then fk20_msm_comb and the graph-captured G1 FFT) as a context that keeps each
  * setup's tables resident and owns its workspace, so proving allocates nothing once the
  * workspace is reserved.  A context is bound to the device current at creation; calls on
- * one context are serialised by the library.  No reference counterpart exists for the
- * context type itself; fk20_setup_load replaces fk20_msm_makelut (fk20.cuh:34;
+ * one context are serialised by the library: host-side by a lock, and on the device by an
+ * event, so a fk20_prove_rows / fk20_prove_batch / fk20_commit_rows call on one stream
+ * starts after the previous such call on another stream has finished with the context's
+ * workspace (fk20_prove_batch_ws, which brings its own workspace, is not ordered).  No
+ * reference counterpart exists for the context type itself; fk20_setup_load replaces
+ * fk20_msm_makelut (fk20.cuh:34;
  * fk20_msm.cu:79) and fk20_prove_rows replaces the fk20_poly2h_fft sequence with
  * fk20_msm_comb (fk20.cuh:35) and fk20_hext_fft_2_h_fft_512 (fk20.cuh:38). */
 typedef struct fk20_ctx fk20_ctx_t;
@@ -128,8 +133,16 @@ int fk20_ctx_set_graph(fk20_ctx_t *ctx, int enable);
  * 64-row butterfly starts as soon as the two butterflies it reads have finished; 0 (default):
  * one launch per stage.  Results are identical; on MI355X the per-stage launches are faster
  * (in the persistent form a wave that has taken a butterfly holds its slot while it waits for
- * the butterfly's inputs, so fewer ladders run at once). */
+ * the butterfly's inputs, so fewer ladders run at once).  Each of those waits is bounded;
+ * with the persistent form on, fk20_prove_rows / fk20_prove_batch synchronise `stream` and
+ * return FK20_ETIMEDOUT if any wait expired (its butterfly then ran on unfinished inputs).
+ * fk20_prove_batch_ws always uses the per-stage launches. */
 int fk20_ctx_set_chain(fk20_ctx_t *ctx, int enable);
+/* Bound of each wait of the persistent form, in ticks of the device's constant-rate wall
+ * clock (100 MHz on MI355X); default 2^27 (~1.3 s, where a legitimate wait lasts at most one
+ * pass, tens of ms).  0 makes every wait that finds its inputs unfinished give up at once:
+ * a test of the FK20_ETIMEDOUT path in which every kernel still exits normally. */
+int fk20_ctx_set_chain_wait(fk20_ctx_t *ctx, uint64_t ticks);
 /* 2 (default): fk20_prove_rows / fk20_prove_batch split the rows into two halves at a 64-row
  * boundary and run the halves' launch sequences on two internal streams forked from and
  * joined back to the caller's stream (captured as such in a hipGraph), so that one half's

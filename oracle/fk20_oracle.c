@@ -22,9 +22,11 @@
  * Field elements are held internally in Montgomery form with 64-bit limbs; all
  * public functions take and return the reference's standard form.
  */
+#define _GNU_SOURCE
 #include "fk20_oracle.h"
 
 #include <pthread.h>
+#include <sched.h>
 #include <stdlib.h>
 #include <string.h>
 #include <unistd.h>
@@ -832,6 +834,8 @@ void or_setup2xext_fft(u64 *xext_fft, const u64 *srs, int N) {
     free(a);
 }
 
+void or_g1_gen_mul_affine_n(u64 *aff, const u64 *scalars, int64_t count, int nthreads);
+
 /* Scalar shadow of xext_fft: xhat_i[m] = tau^(N-l-1-i-l*m) for m <= k-2, else 0; FFT in Fr */
 static void xhat_fft_mont(u64 *xh, const u64 tau[4], int N) {
     const int l = 16, k = N / l, n2 = 2 * k;
@@ -854,20 +858,23 @@ void or_xext_fft_from_tau(u64 *xext_fft, const u64 tau[4], int N) {
     ensure_init();
     const int n2 = N / 8;
     u64 *xh = (u64 *)malloc(sizeof(u64) * 4 * 16 * (size_t)n2);
+    u64 *aff = (u64 *)malloc(sizeof(u64) * 12 * 16 * (size_t)n2);
     xhat_fft_mont(xh, tau, N);
-    u64 g[18];
-    or_g1p_gen(g);
-    gp G;
-    gp_from_std(&G, g);
+    for (int i = 0; i < 16 * n2; i++) from_mont(xh + 4 * (size_t)i, xh + 4 * (size_t)i, &FR);
+    or_g1_gen_mul_affine_n(aff, xh, 16 * (int64_t)n2, 0);
     for (int i = 0; i < 16 * n2; i++) {
-        u64 s[4], p[18];
-        gp t;
-        from_mont(s, xh + 4 * (size_t)i, &FR);
-        gp_mul(&t, &G, s);
-        gp_to_std(p, &t);
-        or_g1p_normalize(xext_fft + 18 * (size_t)i, p);
+        u64 *p = xext_fft + 18 * (size_t)i;
+        const u64 *a = aff + 12 * (size_t)i;
+        if (mp_is_zero(a, 12)) {
+            or_g1p_inf(p);
+            continue;
+        }
+        memcpy(p, a, 96);
+        memset(p + 12, 0, 48);
+        p[12] = 1;
     }
     free(xh);
+    free(aff);
 }
 
 /* ------------------------------------------------------------------------- */
@@ -922,4 +929,251 @@ void or_kzg_quotient_at(u64 out[4], const u64 *poly, int N, const u64 c[4], cons
     from_mont(out, acc, &FR);
     free(rem);
     free(q);
+}
+
+/* ------------------------------------------------------------------------- */
+/* Bulk closed forms: [s]G for many scalars by fixed-base windows over G.
+ *
+ * The scalar-shadow closed form of the FK20 path (h_fft[k] = [s_k]G for a setup with known
+ * secret) needs one multiplication of the generator per output point; these helpers do that
+ * for whole batches.  The scalar is reduced mod r and split into 32 unsigned 8-bit windows;
+ * gen_tab[w][d-1] = d 2^(8w) G (affine, Montgomery), so [s]G is at most 32 mixed additions
+ * (RCB algorithm 8, complete for any accumulator).  The table is built once, by doublings
+ * and additions over G, and converted to affine one point at a time. */
+
+#define GEN_WIN 8
+#define GEN_NWIN 32
+#define GEN_H 255
+
+static u64 (*gen_tab)[GEN_H][12];  /* [GEN_NWIN][GEN_H] x[6] y[6], Montgomery */
+static pthread_once_t gen_once = PTHREAD_ONCE_INIT;
+
+static void gen_tab_init(void) {
+    ensure_init();
+    gen_tab = malloc(sizeof(u64) * 12 * GEN_H * GEN_NWIN);
+    u64 g[18];
+    or_g1p_gen(g);
+    gp base;
+    gp_from_std(&base, g);
+    for (int w = 0; w < GEN_NWIN; w++) {
+        gp acc = base;
+        for (int d = 1; d <= GEN_H; d++) {
+            if (d > 1) gp_add(&acc, &acc, &base);
+            gp_to_affine_m(gen_tab[w][d - 1], gen_tab[w][d - 1] + 6, &acc);
+        }
+        for (int b = 0; b < GEN_WIN; b++) gp_dbl(&base, &base);
+    }
+}
+
+/* [s]G in Montgomery projective form, s any 256-bit value */
+static void gen_mul_m(gp *r, const u64 s[4]) {
+    pthread_once(&gen_once, gen_tab_init);
+    u64 t[4], k[4];
+    to_mont(t, s, &FR);
+    from_mont(k, t, &FR);  /* k = s mod r */
+    gp_inf(r);
+    for (int w = 0; w < GEN_NWIN; w++) {
+        const unsigned d = (unsigned)(k[w / 8] >> (8 * (w % 8))) & 0xffu;
+        if (d) gp_add_aff(r, r, gen_tab[w][d - 1], gen_tab[w][d - 1] + 6);
+    }
+}
+
+/* Canonical affine of n Montgomery points by Montgomery's batch inversion (one Fermat
+ * inversion for the whole run); infinity -> (0, 0). */
+static void gp_to_affine_batch(u64 *aff, const gp *p, int64_t n) {
+    u64 *pre = malloc(sizeof(u64) * 6 * (size_t)(n + 1));
+    memcpy(pre, FP.r1, 48);
+    for (int64_t i = 0; i < n; i++) {
+        const u64 *z = mp_is_zero(p[i].z, 6) ? FP.r1 : p[i].z;
+        fp_m(pre + 6 * (i + 1), pre + 6 * i, z);
+    }
+    u64 inv[6];
+    mont_inv(inv, pre + 6 * n, &FP);
+    for (int64_t i = n - 1; i >= 0; i--) {
+        u64 zi[6], x[6], y[6];
+        const int inf = mp_is_zero(p[i].z, 6);
+        fp_m(zi, inv, pre + 6 * i);
+        if (!inf) fp_m(inv, inv, p[i].z);
+        if (inf) {
+            memset(aff + 12 * i, 0, 96);
+            continue;
+        }
+        fp_m(x, p[i].x, zi);
+        fp_m(y, p[i].y, zi);
+        from_mont(aff + 12 * i, x, &FP);
+        from_mont(aff + 12 * i + 6, y, &FP);
+    }
+    free(pre);
+}
+
+typedef struct {
+    int kind;  /* 0: gen_mul affine, 1: gen_mul projective times z, 2: match, 3: shadow rows,
+                  4: MSM scalar shadow */
+    u64 *out;
+    const u64 *a, *b;
+    const u64 *tau;
+    int N;
+    int64_t i0, i1;
+    int64_t bad, first;
+} bulk_job;
+
+static void *bulk_worker(void *arg) {
+    bulk_job *jb = (bulk_job *)arg;
+    const int64_t n = jb->i1 - jb->i0;
+    if (jb->kind == 0) {
+        gp *tmp = malloc(sizeof(gp) * (size_t)(n > 0 ? n : 1));
+        for (int64_t i = 0; i < n; i++) gen_mul_m(tmp + i, jb->a + 4 * (jb->i0 + i));
+        gp_to_affine_batch(jb->out + 12 * jb->i0, tmp, n);
+        free(tmp);
+    } else if (jb->kind == 1) {
+        for (int64_t i = jb->i0; i < jb->i1; i++) {
+            gp t;
+            u64 zm[6];
+            gen_mul_m(&t, jb->a + 4 * i);
+            to_mont(zm, jb->b + 6 * i, &FP);
+            if (mp_is_zero(t.z, 6)) {
+                /* infinity as (0, z, 0): a non-normalised representative */
+                memset(t.x, 0, 48);
+                memcpy(t.y, zm, 48);
+            } else {
+                fp_m(t.x, t.x, zm);
+                fp_m(t.y, t.y, zm);
+                fp_m(t.z, t.z, zm);
+            }
+            gp_to_std(jb->out + 18 * i, &t);
+        }
+    } else if (jb->kind == 2) {
+        jb->bad = 0;
+        jb->first = -1;
+        for (int64_t i = jb->i0; i < jb->i1; i++) {
+            gp p;
+            u64 ax[6], ay[6], l[6];
+            gp_from_std(&p, jb->a + 18 * i);
+            const u64 *e = jb->b + 12 * i;
+            int ok;
+            if (mp_is_zero(e, 12)) {
+                ok = mp_is_zero(p.z, 6) && mp_is_zero(p.x, 6) && !mp_is_zero(p.y, 6);
+            } else {
+                to_mont(ax, e, &FP);
+                to_mont(ay, e + 6, &FP);
+                ok = !mp_is_zero(p.z, 6);
+                fp_m(l, ax, p.z);
+                ok = ok && memcmp(l, p.x, 48) == 0;
+                fp_m(l, ay, p.z);
+                ok = ok && memcmp(l, p.y, 48) == 0;
+            }
+            if (!ok) {
+                if (jb->first < 0) jb->first = i;
+                jb->bad++;
+            }
+        }
+    } else if (jb->kind == 4) {
+        /* out[r][j] = sum_i tc_fft[r][i][j] * xs[i][j] (fk20_msm.cu:23-66 on scalars) */
+        const int n2 = jb->N;
+        for (int64_t r = jb->i0; r < jb->i1; r++)
+            for (int j = 0; j < n2; j++) {
+                u64 acc[4] = {0, 0, 0, 0};
+                for (int i = 0; i < 16; i++) {
+                    u64 a[4], b[4];
+                    to_mont(a, jb->a + 4 * (((size_t)r * 16 + i) * n2 + j), &FR);
+                    to_mont(b, jb->b + 4 * ((size_t)i * n2 + j), &FR);
+                    mont_mul(a, a, b, &FR);
+                    mod_add(acc, acc, a, &FR);
+                }
+                from_mont(jb->out + 4 * ((size_t)r * n2 + j), acc, &FR);
+            }
+    } else {
+        const int n2 = jb->N / 8;
+        for (int64_t r = jb->i0; r < jb->i1; r++)
+            or_fk20_shadow(jb->out + 4 * (size_t)n2 * r, jb->a + 4 * (size_t)jb->N * r, jb->tau, jb->N);
+    }
+    return NULL;
+}
+
+/* Default worker count of the bulk helpers: the CPUs this process may run on, at most 16 (the
+ * per-GPU CPU share of a GPU box, whatever the machine's total). */
+static int bulk_threads(void) {
+    cpu_set_t set;
+    int n = 0;
+    if (sched_getaffinity(0, sizeof set, &set) == 0) n = CPU_COUNT(&set);
+    if (n <= 0) n = or_default_threads();
+    return n < 16 ? n : 16;
+}
+
+/* Splits [0, count) over nthreads workers of one kind; returns summed mismatches (kind 2). */
+static int64_t bulk_run(bulk_job proto, int64_t count, int nthreads, int64_t *first) {
+    ensure_init();
+    pthread_once(&gen_once, gen_tab_init);
+    if (nthreads <= 0) nthreads = bulk_threads();
+    if (nthreads > count) nthreads = (int)(count > 0 ? count : 1);
+    pthread_t *th = malloc(sizeof(pthread_t) * (size_t)nthreads);
+    bulk_job *jobs = malloc(sizeof(bulk_job) * (size_t)nthreads);
+    for (int t = 0; t < nthreads; t++) {
+        jobs[t] = proto;
+        jobs[t].i0 = count * t / nthreads;
+        jobs[t].i1 = count * (t + 1) / nthreads;
+        if (nthreads == 1) bulk_worker(&jobs[t]);
+        else pthread_create(&th[t], NULL, bulk_worker, &jobs[t]);
+    }
+    if (nthreads > 1)
+        for (int t = 0; t < nthreads; t++) pthread_join(th[t], NULL);
+    int64_t bad = 0, f = -1;
+    for (int t = 0; t < nthreads; t++) {
+        bad += jobs[t].bad;
+        if (f < 0 && jobs[t].first >= 0) f = jobs[t].first;
+    }
+    if (first) *first = f;
+    free(th);
+    free(jobs);
+    return bad;
+}
+
+void or_g1_gen_mul_affine_n(u64 *aff, const u64 *scalars, int64_t count, int nthreads) {
+    bulk_job j;
+    memset(&j, 0, sizeof j);
+    j.kind = 0;
+    j.out = aff;
+    j.a = scalars;
+    bulk_run(j, count, nthreads, NULL);
+}
+
+void or_g1_gen_mul_proj_n(u64 *proj, const u64 *scalars, const u64 *zs, int64_t count, int nthreads) {
+    bulk_job j;
+    memset(&j, 0, sizeof j);
+    j.kind = 1;
+    j.out = proj;
+    j.a = scalars;
+    j.b = zs;
+    bulk_run(j, count, nthreads, NULL);
+}
+
+int64_t or_g1p_match_affine_n(const u64 *p, const u64 *aff, int64_t count, int nthreads, int64_t *first) {
+    bulk_job j;
+    memset(&j, 0, sizeof j);
+    j.kind = 2;
+    j.a = p;
+    j.b = aff;
+    return bulk_run(j, count, nthreads, first);
+}
+
+void or_fk20_shadow_rows(u64 *s_fft, const u64 *poly, const u64 tau[4], int N, int rows, int nthreads) {
+    bulk_job j;
+    memset(&j, 0, sizeof j);
+    j.kind = 3;
+    j.out = s_fft;
+    j.a = poly;
+    j.tau = tau;
+    j.N = N;
+    bulk_run(j, rows, nthreads, NULL);
+}
+
+void or_fk20_msm_shadow(u64 *out, const u64 *tc_fft, const u64 *xs, int n2, int rows, int nthreads) {
+    bulk_job j;
+    memset(&j, 0, sizeof j);
+    j.kind = 4;
+    j.out = out;
+    j.a = tc_fft;
+    j.b = xs;
+    j.N = n2;
+    bulk_run(j, rows, nthreads, NULL);
 }

@@ -96,6 +96,25 @@ void or_fk20_shadow(uint64_t *s_fft, const uint64_t *poly, const uint64_t tau[4]
 void or_kzg_quotient_at(uint64_t out[4], const uint64_t *poly, int N, const uint64_t c[4],
                         const uint64_t tau[4]);
 
+/* ---- Bulk closed forms (threaded; nthreads <= 0: all online CPUs) ---- */
+/* aff[i] = canonical affine [s_i] G (infinity (0,0)); s_i any 256-bit value. */
+void or_g1_gen_mul_affine_n(uint64_t *aff, const uint64_t *scalars, int64_t count, int nthreads);
+/* proj[i] = [s_i] G as the projective representative (z_i X, z_i Y, z_i Z) of the fixed-base
+ * sum (X, Y, Z), z_i a non-zero standard-form Fp element; infinity becomes (0, z_i, 0). */
+void or_g1_gen_mul_proj_n(uint64_t *proj, const uint64_t *scalars, const uint64_t *zs, int64_t count,
+                          int nthreads);
+/* Number of i with p[i] (projective, any representative) != aff[i] (affine, (0,0) = infinity);
+ * *first receives the smallest such i (or -1). */
+int64_t or_g1p_match_affine_n(const uint64_t *p, const uint64_t *aff, int64_t count, int nthreads,
+                              int64_t *first);
+/* or_fk20_shadow of `rows` polynomials [rows][N] -> s_fft[rows][N/8]. */
+void or_fk20_shadow_rows(uint64_t *s_fft, const uint64_t *poly, const uint64_t tau[4], int N, int rows,
+                         int nthreads);
+/* Scalar shadow of fk20_msm: out[rows][n2] = sum_i tc_fft[rows][i][n2] * xs[i][n2] over Fr
+ * (the MSM of bases [xs]G is [out]G). */
+void or_fk20_msm_shadow(uint64_t *out, const uint64_t *tc_fft, const uint64_t *xs, int n2, int rows,
+                        int nthreads);
+
 /* Number of threads or_fk20_poly2h_fft would use for nthreads <= 0 (online CPUs). */
 int or_default_threads(void);
 

@@ -33,6 +33,7 @@ def lib():
             build()
         _lib = ctypes.CDLL(LIBPATH)
         _lib.or_default_threads.restype = ctypes.c_int
+        _lib.or_g1p_match_affine_n.restype = ctypes.c_int64
         for fn in ["or_fr_eq", "or_fp_eq", "or_g1p_is_inf", "or_g1p_is_point", "or_g1p_eq"]:
             getattr(_lib, fn).restype = ctypes.c_int
     return _lib
@@ -257,6 +258,81 @@ def kzg_quotient_at(poly, c, tau):
 
 def default_threads():
     return int(lib().or_default_threads())
+
+
+def pool_threads():
+    """Worker threads for the bulk checkers: the CPUs this process may run on, at most 16 (a
+    GPU box's per-GPU CPU share, whatever the machine's total)."""
+    try:
+        n = len(os.sched_getaffinity(0))
+    except AttributeError:
+        n = os.cpu_count() or 1
+    return max(1, min(16, n))
+
+
+# ---- bulk closed forms ----
+
+def gen_mul_affine(scalars, nthreads=0):
+    """Canonical affine [s]G (shape (..., 12), infinity (0, 0)) for fr scalars (..., 4), by
+    fixed-base windows over G."""
+    s = np.ascontiguousarray(scalars, dtype=np.uint64)
+    n = s.size // 4
+    out = np.zeros(s.shape[:-1] + (12,), dtype=np.uint64)
+    lib().or_g1_gen_mul_affine_n(_p(out), _p(s), ctypes.c_int64(n), ctypes.c_int(nthreads or pool_threads()))
+    return out
+
+
+def gen_mul_projective(scalars, zs, nthreads=0):
+    """[s]G as a non-normalised projective representative: the fixed-base sum's (X, Y, Z)
+    scaled by z (infinity (0, z, 0)); zs are non-zero standard-form Fp elements (..., 6)."""
+    s = np.ascontiguousarray(scalars, dtype=np.uint64)
+    z = np.ascontiguousarray(zs, dtype=np.uint64)
+    n = s.size // 4
+    assert z.size // 6 == n
+    out = np.zeros(s.shape[:-1] + (18,), dtype=np.uint64)
+    lib().or_g1_gen_mul_proj_n(_p(out), _p(s), _p(z), ctypes.c_int64(n), ctypes.c_int(nthreads or pool_threads()))
+    return out
+
+
+def match_affine(proj, aff, nthreads=0):
+    """(mismatches, first mismatching index or -1) of projective points (..., 18) against
+    expected affine points (..., 12), compared by cross-multiplication."""
+    p = np.ascontiguousarray(proj, dtype=np.uint64)
+    a = np.ascontiguousarray(aff, dtype=np.uint64)
+    n = p.size // 18
+    assert a.size // 12 == n
+    first = ctypes.c_int64()
+    bad = lib().or_g1p_match_affine_n(_p(p), _p(a), ctypes.c_int64(n), ctypes.c_int(nthreads or pool_threads()),
+                                      ctypes.byref(first))
+    return int(bad), int(first.value)
+
+
+def fk20_shadow_rows(poly, tau, nthreads=0):
+    """Scalar shadow s_fft (rows, N/8, 4) of polynomials (rows, N, 4): h_fft[r][k] = [s[r][k]]G."""
+    poly = np.ascontiguousarray(poly, dtype=np.uint64)
+    rows, N = poly.shape[0], poly.shape[1]
+    out = np.zeros((rows, N // 8, 4), dtype=np.uint64)
+    lib().or_fk20_shadow_rows(_p(out), _p(poly), _p(fr(tau)), ctypes.c_int(N), ctypes.c_int(rows),
+                              ctypes.c_int(nthreads or pool_threads()))
+    return out
+
+
+def msm_shadow(tc_fft, xs, nthreads=0):
+    """Scalars of fk20_msm over bases [xs]G: out (rows, n2, 4) = sum_i tc_fft[:, i, :] * xs[i]."""
+    tc = np.ascontiguousarray(tc_fft, dtype=np.uint64)
+    xs = np.ascontiguousarray(xs, dtype=np.uint64)
+    rows, n2 = tc.shape[0], tc.shape[2]
+    assert tc.shape[1] == 16 and xs.shape[:2] == (16, n2)
+    out = np.zeros((rows, n2, 4), dtype=np.uint64)
+    lib().or_fk20_msm_shadow(_p(out), _p(tc), _p(xs), ctypes.c_int(n2), ctypes.c_int(rows),
+                             ctypes.c_int(nthreads or pool_threads()))
+    return out
+
+
+def fk20_closed_form_affine(poly, tau, nthreads=0):
+    """Expected h_fft of every row in canonical affine form (rows, N/8, 12) for a setup with
+    secret tau: [s_k]G of the scalar shadow."""
+    return gen_mul_affine(fk20_shadow_rows(poly, tau, nthreads), nthreads)
 
 
 # ---- output formats ----

@@ -188,3 +188,56 @@ def test_two_stream_split_equals_single_stream(prover):
         assert np.array_equal(a, b)
     for sid, _ in groups:
         prover.unload_setup(sid)
+
+
+def test_chain_expired_wait_is_an_error(prover):
+    """A zero wait budget makes every wait of the persistent launch that finds its inputs
+    unfinished give up at once (each wave then runs its butterfly and exits normally); the
+    call must report FK20_ETIMEDOUT instead of returning the stale proofs as valid.  With the
+    default budget restored the same context proves correctly again."""
+    from fk20_amd.lib import Fk20Error
+    N, rows = 1024, 128
+    tau = S.random_secret(411)
+    sid = prover.load_setup(O.xext_fft_from_tau(tau, N), N, 8)
+    poly = S.random_polys(rows, N, 412)
+    try:
+        prover.set_chain(True)
+        prover.set_chain_wait(0)
+        with pytest.raises(Fk20Error) as ei:
+            prover.prove(sid, poly)
+        assert ei.value.status == -5
+        prover.set_chain_wait(1 << 27)
+        got = prover.prove(sid, poly)
+    finally:
+        prover.set_chain_wait(1 << 27)
+        prover.set_chain(False)
+    for r in (0, rows - 1):
+        assert np.array_equal(aff(got[r]), aff(shadow_points(poly[r], tau)))
+    prover.unload_setup(sid)
+
+
+def test_calls_on_two_streams_share_the_workspace_in_order(prover):
+    """Two fk20_prove_rows calls on different streams, enqueued back to back with no host
+    synchronisation between them, use the context's one workspace: the second must wait for
+    the first on the device.  Small batches (one 64-row group: no internal split) are the case
+    where both run on the callers' streams directly."""
+    from fk20_amd.lib import DeviceBuffer, Stream
+    N, rows = 1024, 48
+    tau = S.random_secret(421)
+    sid = prover.load_setup(O.xext_fft_from_tau(tau, N), N, 8)
+    polys = [S.random_polys(rows, N, 422 + k) for k in range(3)]
+    streams = [Stream(), Stream()]
+    pbs = [DeviceBuffer.from_array(p) for p in polys]
+    hbs = [DeviceBuffer(rows * (N // 8) * 144) for _ in polys]
+    for k in range(3):
+        prover.prove_device(sid, hbs[k], pbs[k], rows, stream=streams[k % 2].handle)
+    for st in streams:
+        st.synchronize()
+    outs = [hb.download((rows, N // 8, 18)) for hb in hbs]
+    for b in pbs + hbs:
+        b.free()
+    for st in streams:
+        st.destroy()
+    for poly, got in zip(polys, outs):
+        assert O.match_affine(got, O.fk20_closed_form_affine(poly, tau)) == (0, -1)
+    prover.unload_setup(sid)

@@ -148,3 +148,26 @@ def test_setup2xext_fft():
     srs = O.setup_srs(tau, N + 1)
     srsn, _ = g1_reencode(srs, 17)
     assert np.array_equal(aff(ops.setup2xext_fft(srsn, N)), aff(O.xext_fft_from_tau(tau, N)))
+
+
+@pytest.mark.parametrize("window", [8, 9, 13])
+def test_commit_non_canonical_coefficients(window):
+    """fk20_commit_rows with coefficients re-encoded as v + r and the edge values r, 2^255,
+    2^256 - 1 (their residues mod r are what the commitment must use).  At window 8 the signed
+    windows end exactly at bit 256, so an unreduced value >= 2^255 would carry out of them."""
+    N, rows = 1024, 3
+    tau = S.random_secret(7400 + window)
+    srs = O.setup_srs(tau, N)
+    poly = S.random_polys(rows, N, 7401)
+    enc = fr_plus_r(poly)
+    edge = [R, 1 << 255, (1 << 256) - 1, R - 1, (1 << 256) - R]
+    for k, v in enumerate(edge):
+        enc[2, k] = O.int_to_limbs(v, 4)
+        poly[2, k] = O.int_to_limbs(v % R, 4)
+    with Prover() as p:
+        sid = p.load_commit_setup(srs, N, window)
+        got = p.commit(sid, enc)
+        want = p.commit(sid, poly)
+    assert np.array_equal(aff(got), aff(want))
+    for r in range(rows):
+        assert np.array_equal(aff(got[r]), aff(O.commit_closed_form(poly[r], tau))), r

@@ -372,3 +372,81 @@ def test_check_proof_multi_on_oracle_proofs():
         assert PR.check_proof_multi(commitment, proof, x, ys, srs_aff, g2_s16, O.g1_affine_add, O.g1_affine_mul)
         ys[5] = (ys[5] + 1) % PR.R
         assert not PR.check_proof_multi(commitment, proof, x, ys, srs_aff, g2_s16, O.g1_affine_add, O.g1_affine_mul)
+
+
+# ---------------------------------------------------------------- bulk closed forms
+
+def test_gen_mul_bulk_matches_double_and_add():
+    """Fixed-base windows over G (or_g1_gen_mul_affine_n) against MSB-first double-and-add
+    (or_g1p_mul, pinned by the G1 KATs) on random and edge scalars: 0, 1, r - 1, r, r + 1,
+    2^255, 2^256 - 1 (the window split must reduce mod r first)."""
+    G = O.g1p_gen()
+    edge = [0, 1, R - 1, R, R + 1, 1 << 255, (1 << 256) - 1, 0xFF, 0x100, 0xFF << 248]
+    ks = np.concatenate([S.random_fr(60, 71), np.array([O.fr(v % (1 << 256)) if v < (1 << 256) else O.fr(0)
+                                                         for v in edge])])
+    got = O.gen_mul_affine(ks, nthreads=3)
+    exp = np.array([O.to_affine(O.g1p_op("mul", G, k)) for k in ks])
+    assert np.array_equal(got, exp)
+    assert not got[60].any() and not got[63].any()  # 0 G and r G are infinity
+
+
+def test_gen_mul_projective_representatives():
+    ks = S.random_fr(40, 72)
+    ks[5] = 0
+    zs = S.random_fp_nonzero(40, 73)
+    pts = O.gen_mul_projective(ks, zs, nthreads=2)
+    assert all(O.g1p_is_point(p) for p in pts)
+    assert O.g1p_is_inf(pts[5]) and O.limbs_to_int(pts[5, 6:12]) == O.limbs_to_int(zs[5]) % P
+    assert np.array_equal(O.to_affine(pts), O.gen_mul_affine(ks))
+    # not normalised: z != 1, and scaling by a different z gives other coordinates
+    assert all(O.limbs_to_int(p[12:]) not in (0, 1) for i, p in enumerate(pts) if i != 5)
+    other = O.gen_mul_projective(ks, S.random_fp_nonzero(40, 83), nthreads=2)
+    assert not np.array_equal(other[0], pts[0]) and O.g1p_eq(other[0], pts[0])
+
+
+def test_match_affine_catches_planted_differences():
+    ks = S.random_fr(30, 74)
+    ks[3] = 0
+    exp = O.gen_mul_affine(ks)
+    pts = O.gen_mul_projective(ks, S.random_fp_nonzero(30, 75))
+    assert O.match_affine(pts, exp) == (0, -1)
+    for idx, word in ((7, 0), (9, 8), (11, 14)):   # x, y, z words
+        bad = pts.copy()
+        bad[idx, word] ^= np.uint64(1 << 20)
+        assert O.match_affine(bad, exp) == (1, idx)
+    neg = pts.copy()                               # -P: same x, other y
+    neg[12] = O.g1p_op("neg", pts[12])
+    assert O.match_affine(neg, exp) == (1, 12)
+    inf_exp = exp.copy()                           # expected infinity, got a point
+    inf_exp[4] = 0
+    assert O.match_affine(pts, inf_exp) == (1, 4)
+    got_inf = pts.copy()                           # expected a point, got infinity
+    got_inf[6] = O.g1p_inf()
+    assert O.match_affine(got_inf, exp) == (1, 6)
+
+
+def test_shadow_rows_and_msm_shadow():
+    """The threaded shadow equals the per-row one, and the MSM shadow's [s]G equals the
+    oracle's double-and-add fk20_msm over bases [x]G."""
+    tau = S.random_secret(76)
+    poly = S.random_polys(5, 1024, 77)
+    rows = O.fk20_shadow_rows(poly, tau, nthreads=3)
+    for r in range(5):
+        assert np.array_equal(rows[r], O.fk20_shadow(poly[r], tau))
+    n2 = 128
+    tc = S.random_fr(2 * 16 * n2, 78).reshape(2, 16, n2, 4)
+    xs = S.random_fr(16 * n2, 79).reshape(16, n2, 4)
+    xext = O.gen_mul_projective(xs, S.random_fp_nonzero(16 * n2, 80).reshape(16, n2, 6))
+    sc = O.msm_shadow(tc, xs)
+    assert O.match_affine(O.msm(tc[1], xext), O.gen_mul_affine(sc[1])) == (0, -1)
+    # a transposed operand (tc[:, :, j] against xs[j, i]) must not match
+    swapped = O.msm_shadow(tc, np.ascontiguousarray(np.roll(xs, 1, axis=0)))
+    assert not np.array_equal(swapped, sc)
+
+
+def test_closed_form_affine_matches_oracle_rows():
+    tau = S.random_secret(81)
+    poly = S.random_polys(2, 2048, 82)
+    xext = O.xext_fft_from_tau(tau, 2048)
+    got = O.poly2h_fft(poly, xext, nthreads=2)
+    assert O.match_affine(got, O.fk20_closed_form_affine(poly, tau)) == (0, -1)
