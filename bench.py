@@ -249,18 +249,23 @@ def time_f1(b, prover, sid, reps):
 
 def time_uneven(prover, rows_each, window, steps, seed):
     """512 rows each of N = 1024 / 2048 / 4096 (three setups) in one fk20_prove_batch pass;
-    each setup uses its autotuned window (tuned.json) or `window`."""
+    each setup uses its autotuned window (tuned.json) or `window`.  Returns the pass times and,
+    per group, (N, secret, host polynomials, host h_fft of the last timed pass) for the check
+    against the oracle in the CPU leg."""
     from fk20_amd.autotune import tuned_window
 
-    items, bufs = [], []
+    items, bufs, groups = [], [], []
     for k, N in enumerate((1024, 2048, 4096)):
-        xext = device_setup(N, S.random_secret(seed + 100 + k))
+        tau = S.random_secret(seed + 100 + k)
+        xext = device_setup(N, tau)
         sid = prover.load_setup(xext, N, tuned_window(N, window))
         xext.free()
-        pb = L.DeviceBuffer.from_array(S.random_polys(rows_each, N, seed + 200 + k))
+        poly = S.random_polys(rows_each, N, seed + 200 + k)
+        pb = L.DeviceBuffer.from_array(poly)
         hb = L.DeviceBuffer(rows_each * (N // 8) * 144)
         bufs += [pb, hb]
         items.append((sid, rows_each, pb, hb))
+        groups.append([N, tau, poly, hb])
     stream = L.Stream()
     e0, e1 = L.Event(), L.Event()
     prover.prove_batch_device(items, stream=stream.handle)
@@ -271,6 +276,9 @@ def time_uneven(prover, rows_each, window, steps, seed):
         prover.prove_batch_device(items, stream=stream.handle)
         e1.record(stream)
         ms.append(e0.elapsed_ms(e1))
+    stream.synchronize()
+    for g in groups:
+        g[3] = g[3].download((rows_each, g[0] // 8, 18))
     for sid, _, _, _ in items:
         prover.unload_setup(sid)
     for x in bufs:
@@ -278,7 +286,7 @@ def time_uneven(prover, rows_each, window, steps, seed):
     e0.destroy()
     e1.destroy()
     stream.destroy()
-    return ms
+    return ms, groups
 
 
 # ---------------------------------------------------------------- C5: serving loop
@@ -570,25 +578,68 @@ def bench_serve(args):
 
 # ---------------------------------------------------------------- CPU baseline
 
-def cpu_baseline(b, sample_rows, threads):
-    """Oracle on host cores over the first sample_rows rows; checks those GPU rows bit-exactly."""
+def host_cpus():
+    """(CPUs this process may run on, CPUs the machine reports, CPU model name)."""
+    try:
+        avail = len(os.sched_getaffinity(0))
+    except AttributeError:
+        avail = os.cpu_count() or 1
+    model = None
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return avail, os.cpu_count(), model
+
+
+def cpu_baseline(b, threads, runs, gpu_out, uneven_groups):
+    """The CPU leg: the oracle's restatement of FK20Py fk20_multi (double-and-add G1, one row
+    per thread) timed on the host cores, `runs` runs of `threads` rows each (median run), and
+    the bit-exactness checks of this run's GPU outputs: every row of the timed batch and of
+    the uneven batch against the scalar-shadow closed form, and the CPU rows against the same
+    GPU rows."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle as O
 
     O.lib()
-    t0 = time.perf_counter()
-    exp = O.poly2h_fft(b.poly_host[:sample_rows], b.xext_host, nthreads=threads)
-    el = time.perf_counter() - t0
-    got = b.h_fft.download((b.rows, b.n2, 18))[:sample_rows]
-    same = np.array_equal(O.to_affine(got.reshape(-1, 18)), O.to_affine(exp.reshape(-1, 18)))
+    avail, total, model = host_cpus()
+    if threads <= 0:
+        threads = max(1, min(16, avail))  # the per-GPU CPU share of a GPU box
+    runs = max(1, min(runs, b.rows // threads))
+    times, cpu_rows = [], []
+    for k in range(runs):
+        lo = k * threads
+        t0 = time.perf_counter()
+        cpu_rows.append(O.poly2h_fft(b.poly_host[lo:lo + threads], b.xext_host, nthreads=threads))
+        times.append(time.perf_counter() - t0)
+    el = statistics.median(times)
+    checked = runs * threads
+    cpu_same = O.match_affine(gpu_out[:checked], O.to_affine(np.concatenate(cpu_rows).reshape(-1, 18)))[0] == 0
+    bad, first = O.match_affine(gpu_out, O.fk20_closed_form_affine(b.poly_host, b.tau))
+    exact = {"batch_rows": b.rows, "batch_points_differing": bad, "cpu_rows_equal_gpu": cpu_same}
+    if uneven_groups:
+        exact["uneven_rows"] = 0
+        exact["uneven_points_differing"] = 0
+        for N, tau, poly, out in uneven_groups:
+            exact["uneven_rows"] += poly.shape[0]
+            exact["uneven_points_differing"] += O.match_affine(out, O.fk20_closed_form_affine(poly, tau))[0]
+    ok = bad == 0 and cpu_same and not exact.get("uneven_points_differing")
     return {
-        "value": round(sample_rows / el, 4),
+        "value": round(threads / el, 4),
         "unit": "rows/s",
         "cores": threads,
         "kind": "port",
-        "sample": "%d rows of N=%d from the timed batch, one oracle row per thread (%.1f s)"
-                  % (sample_rows, b.N, el),
-    }, same
+        "sample": "%d runs x %d rows of N=%d (%d of the %d timed rows, one oracle row per thread), "
+                  "median run %.1f s" % (runs, threads, b.N, checked, b.rows, el),
+        "run_s": [round(t, 2) for t in times],
+        "host_cpus_available": avail,
+        "host_cpus_total": total,
+        "cpu_model": model,
+    }, exact, ok
 
 
 def main():
@@ -603,8 +654,9 @@ def main():
     ap.add_argument("--seed", type=int, default=2024)
     ap.add_argument("--plain-steps", type=int, default=2, help="steps of the plain P2 form (0: skip)")
     ap.add_argument("--p1-rows", type=int, default=4, help="rows timed for the per-row form P1 (0: skip)")
-    ap.add_argument("--cpu-rows", type=int, default=16, help="rows in the CPU baseline sample")
-    ap.add_argument("--cpu-threads", type=int, default=16)
+    ap.add_argument("--cpu-runs", type=int, default=3, help="CPU baseline runs (one row per thread each)")
+    ap.add_argument("--cpu-threads", type=int, default=0,
+                    help="CPU baseline threads (0: the CPUs this process may use, at most 16)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--streams", type=int, default=2,
                     help="launch sequences per batch: 2 halves on two streams (default) or 1")
@@ -770,18 +822,19 @@ def main():
     if args.p1_rows:
         f1 = time_f1(b, prover, sid, max(3, args.p1_rows))
         forms["F_1row_latency_ms_p50"] = round(statistics.median(f1), 3)
-    uneven = None
+    # the F output of the last timed step (the graph replays rewrote it)
+    gpu_out = b.h_fft.download((b.rows, b.n2, 18))
+    uneven, uneven_groups = None, None
     if args.uneven_rows:
         log("timing uneven batch: %d rows each of N=1024/2048/4096 in one pass" % args.uneven_rows)
-        um = time_uneven(prover, args.uneven_rows, args.window, max(2, args.steps // 2), args.seed)
+        um, uneven_groups = time_uneven(prover, args.uneven_rows, args.window, max(2, args.steps // 2), args.seed)
         uneven = {"rows": 3 * args.uneven_rows, "ms_p50": round(statistics.median(um), 3),
                   "rows_per_s": round(3 * args.uneven_rows / (statistics.median(um) / 1e3), 1)}
 
-    cpu, exact = None, None
+    cpu, exact, ok = None, None, None
     if not args.no_cpu:
-        # the F output of the last timed step is in b.h_fft (the graph replays rewrote it)
-        log("cpu baseline: oracle over %d rows with %d threads" % (args.cpu_rows, args.cpu_threads))
-        cpu, exact = cpu_baseline(b, args.cpu_rows, args.cpu_threads)
+        log("cpu leg: oracle baseline (%d runs) and bit-exact checks of every GPU row" % args.cpu_runs)
+        cpu, exact, ok = cpu_baseline(b, args.cpu_threads, args.cpu_runs, gpu_out, uneven_groups)
 
     out = {
         "metric": METRIC,
@@ -807,12 +860,12 @@ def main():
         "setup_table_build_s": round(setup_s, 3),
         "roofline": roofline,
         "cpu_baseline": cpu,
-        "bit_exact_rows_checked": (args.cpu_rows if cpu else 0),
-        "bit_exact": exact,
+        "bit_exact": ok,
+        "bit_exact_detail": exact,
     }
     print(json.dumps(out), flush=True)
     prover.close()
-    if exact is False:
+    if ok is False:
         sys.exit("GPU rows differ from the oracle")
 
 
