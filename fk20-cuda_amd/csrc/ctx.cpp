@@ -54,14 +54,20 @@ std::vector<int> wnaf(u128 k, int W) {
     return d;
 }
 
-// Ladder cost of a program over digit strings d1, d2 in Fp multiplications: one doubling (8)
-// per position below the top, one complete addition (12) per non-zero digit after the first.
-int program_cost(const std::vector<int> &d1, const std::vector<int> &d2) {
+// Ladder cost of a program over digit strings d1, d2, in 32 x 32-bit multiply-adds of the
+// 14 x 29-bit field arithmetic (a product 392, a square 301, a sum of two products sharing
+// one reduction 588): one Jacobian doubling (3 squares, 2 products, 1 pair: 2275) per position
+// below the top, one mixed addition (4 squares, 5 products, 1 pair: 3752) per non-zero digit
+// after the first, and one product (x times beta or beta^2) per digit on phi(P) or phi^2(P).
+int program_cost(const std::vector<int> &d1, const std::vector<int> &d2, int t) {
     const int L = (int)std::max(d1.size(), d2.size());
-    int nz = 0;
+    int nz = 0, nphi = 0;
     for (int x : d1) nz += x != 0;
     for (int x : d2) nz += x != 0;
-    return std::max(L - 1, 0) * 8 + std::max(nz - 1, 0) * 12;
+    for (int part = 0; part < 2; part++)
+        if ((t + part) % 3 != 0)
+            for (int x : part ? d2 : d1) nphi += x != 0;
+    return std::max(L - 1, 0) * 2275 + std::max(nz - 1, 0) * 3752 + nphi * 392;
 }
 
 // lambda (the eigenvalue of phi on G1) in Montgomery form
@@ -76,12 +82,49 @@ fs lambda_mont() {
     return lam;
 }
 
-// k * P as a program over the wNAF digits of a split of k.  Twelve splits are tried and the
-// cheapest kept: k lambda^-t for t = 0, 1, 2 (then k1's digits act on phi^t(P) and k2's on
-// phi^(t+1)(P)), each as itself or as r minus it (then every digit negated, since
-// (r - k) P = -k P), each with 0 <= k1 < lambda or with k1 - lambda and k2 + 1.  The choice
-// matters most for w^-128 = -w^128 = -(z lambda + z), whose halves are 64-bit.
-void append_program(std::vector<uint32_t> &ops, const uint32_t *k, int W) {
+bool fs_equal(const fs &a, const fs &b) { return memcmp(a.l, b.l, 32) == 0; }
+
+// Evaluates a program on the scalar 1 with phi = lambda (Montgomery Fr): it must give k, and
+// the ladder's incomplete formulas (ladder.hip.h) must meet no exceptional case for an input
+// of order r: before each mixed addition the accumulator a P and the entry b P satisfy
+// a != 0 and a != +-b (mod r).  The table's co-Z additions 2P + (2i - 1)P are never
+// exceptional (|2 -+ (2i - 1)| < 2^W is not a multiple of r).
+bool check_program(const uint32_t *ops, const fs &expect_mont) {
+    fs lam, lam2, acc, one, zero, neg;
+    lam = lambda_mont();
+    fs_mul(lam2, lam, lam);
+    fs_set(one, FR_ONE);
+    fs_zero(acc);
+    fs_zero(zero);
+    bool started = false;
+    for (int i = 0;; i++) {
+        const uint32_t op = ops[i];
+        for (uint32_t d = 0; d < (op & TWOP_DBL_MASK); d++) fs_add(acc, acc, acc);
+        if (op & TWOP_END) break;
+        fs term, m;
+        fs_zero(m);
+        m.l[0] = 2 * ((op >> TWOP_ENTRY_SHIFT) & 7) + 1;
+        fs_to_mont(term, m);
+        if (op & TWOP_PHI2) fs_mul(term, term, lam2);
+        else if (op & TWOP_PHI) fs_mul(term, term, lam);
+        if (op & TWOP_NEG) fs_sub(term, zero, term);
+        if (started) {
+            fs_sub(neg, zero, term);
+            if (fs_equal(acc, zero) || fs_equal(acc, term) || fs_equal(acc, neg)) return false;
+        }
+        fs_add(acc, acc, term);
+        started = true;
+    }
+    return started && fs_equal(acc, expect_mont);
+}
+
+// k * P as a program over the wNAF digits of a split of k; appends it to `ops` and returns
+// true, or returns false if no split passes check_program.  Twelve splits are candidates:
+// k lambda^-t for t = 0, 1, 2 (then k1's digits act on phi^t(P) and k2's on phi^(t+1)(P)),
+// each as itself or as r minus it (then every digit negated, since (r - k) P = -k P), each
+// with 0 <= k1 < lambda or with k1 - lambda and k2 + 1; the cheapest that passes is kept.
+// The choice matters most for w^-128 = -w^128 = -(z lambda + z), whose halves are 64-bit.
+bool append_program(std::vector<uint32_t> &ops, const uint32_t *k, int W, const fs &expect_mont) {
     const u128 lam = ((u128)GLV_LAMBDA[1] << 64) | GLV_LAMBDA[0];
     const fs lam_m = lambda_mont();
     fs lam_inv;  // lambda^-1 = lambda^2
@@ -90,8 +133,11 @@ void append_program(std::vector<uint32_t> &ops, const uint32_t *k, int W) {
     fs_zero(km);
     for (int i = 0; i < 8; i++) km.l[i] = k[i];
     fs_to_mont(km, km);
-    std::vector<int> b1, b2;
-    int best = -1, best_t = 0;
+    struct Cand {
+        int cost, t;
+        std::vector<int> d1, d2;
+    };
+    std::vector<Cand> cands;
     kt = km;
     for (int t = 0; t < 3; t++) {
         if (t) fs_mul(kt, kt, lam_inv);
@@ -117,66 +163,50 @@ void append_program(std::vector<uint32_t> &ops, const uint32_t *k, int W) {
             for (int shifted = 0; shifted < 2; shifted++) {
                 // shifted: k1 - lambda < 0 and k2 + 1
                 if (shifted && k1 == 0) continue;
-                std::vector<int> d1 = wnaf(shifted ? lam - k1 : k1, W), d2 = wnaf(shifted ? k2 + 1 : k2, W);
-                if (shifted) for (int &x : d1) x = -x;
+                Cand c;
+                c.t = t;
+                c.d1 = wnaf(shifted ? lam - k1 : k1, W);
+                c.d2 = wnaf(shifted ? k2 + 1 : k2, W);
+                if (shifted) for (int &x : c.d1) x = -x;
                 if (neg) {
-                    for (int &x : d1) x = -x;
-                    for (int &x : d2) x = -x;
+                    for (int &x : c.d1) x = -x;
+                    for (int &x : c.d2) x = -x;
                 }
-                const int c = program_cost(d1, d2);
-                if (best < 0 || c < best) {
-                    best = c;
-                    best_t = t;
-                    b1 = d1;
-                    b2 = d2;
-                }
+                c.cost = program_cost(c.d1, c.d2, t);
+                cands.push_back(c);
             }
         }
     }
-    const int L = (int)std::max(b1.size(), b2.size());
-    bool started = false;
-    uint32_t pending = 0;
-    for (int t = L - 1; t >= 0; t--) {
-        if (started) pending++;
-        for (int part = 0; part < 2; part++) {
-            const std::vector<int> &d = part ? b2 : b1;
-            const int digit = t < (int)d.size() ? d[t] : 0;
-            if (!digit) continue;
-            const uint32_t e = (uint32_t)((digit < 0 ? -digit : digit) - 1) / 2;
-            const int power = (best_t + part) % 3;  // phi^power of P
-            const uint32_t phi = power == 1 ? TWOP_PHI : power == 2 ? TWOP_PHI2 : 0u;
-            ops.push_back(pending | phi | (e << TWOP_ENTRY_SHIFT) | (digit < 0 ? TWOP_NEG : 0u));
-            pending = 0;
-            started = true;
+    std::stable_sort(cands.begin(), cands.end(), [](const Cand &a, const Cand &b) { return a.cost < b.cost; });
+    for (const Cand &c : cands) {
+        std::vector<uint32_t> prog;
+        const int L = (int)std::max(c.d1.size(), c.d2.size());
+        bool started = false;
+        uint32_t pending = 0;
+        for (int t = L - 1; t >= 0; t--) {
+            if (started) pending++;
+            for (int part = 0; part < 2; part++) {
+                const std::vector<int> &d = part ? c.d2 : c.d1;
+                const int digit = t < (int)d.size() ? d[t] : 0;
+                if (!digit) continue;
+                const uint32_t e = (uint32_t)((digit < 0 ? -digit : digit) - 1) / 2;
+                const int power = (c.t + part) % 3;  // phi^power of P
+                const uint32_t phi = power == 1 ? TWOP_PHI : power == 2 ? TWOP_PHI2 : 0u;
+                prog.push_back(pending | phi | (e << TWOP_ENTRY_SHIFT) | (digit < 0 ? TWOP_NEG : 0u));
+                pending = 0;
+                started = true;
+            }
+        }
+        prog.push_back(TWOP_END | pending);
+        if (check_program(prog.data(), expect_mont)) {
+            ops.insert(ops.end(), prog.begin(), prog.end());
+            return true;
         }
     }
-    ops.push_back(TWOP_END | pending);
+    return false;
 }
 
-// Evaluates a program on the scalar 1 with phi = lambda (Montgomery Fr): must give k.
-bool check_program(const uint32_t *ops, const fs &expect_mont) {
-    fs lam, lam2, acc, one;
-    lam = lambda_mont();
-    fs_mul(lam2, lam, lam);
-    fs_set(one, FR_ONE);
-    fs_zero(acc);
-    for (int i = 0;; i++) {
-        const uint32_t op = ops[i];
-        for (uint32_t d = 0; d < (op & TWOP_DBL_MASK); d++) fs_add(acc, acc, acc);
-        if (op & TWOP_END) break;
-        fs term, m;
-        fs_zero(m);
-        m.l[0] = 2 * ((op >> TWOP_ENTRY_SHIFT) & 7) + 1;
-        fs_to_mont(term, m);
-        if (op & TWOP_PHI2) fs_mul(term, term, lam2);
-        else if (op & TWOP_PHI) fs_mul(term, term, lam);
-        if (op & TWOP_NEG) fs_sub(acc, acc, term);
-        else fs_add(acc, acc, term);
-    }
-    return memcmp(acc.l, expect_mont.l, 32) == 0;
-}
-
-// Builds the programs of w^idx and w^-idx (w of order 512) and checks each on the host.
+// Builds the programs of w^idx and w^-idx (w of order 512), each checked on the host.
 bool build_programs_host(std::vector<uint32_t> &ops, std::vector<uint32_t> &index) {
     DevTables T;
     build_host_tables(T);
@@ -186,10 +216,9 @@ bool build_programs_host(std::vector<uint32_t> &ops, std::vector<uint32_t> &inde
         for (int idx = 0; idx < 512; idx++) {
             const int e = dir == TW_FWD ? idx : (512 - idx) % 512;
             index[dir * 512 + idx] = (uint32_t)ops.size();
-            append_program(ops, T.tw_std[e], TW_WNAF);
             fs expect;
             fs_set(expect, T.tw_mont[e]);
-            if (!check_program(ops.data() + index[dir * 512 + idx], expect)) return false;
+            if (!append_program(ops, T.tw_std[e], TW_WNAF, expect)) return false;
         }
     }
     return true;
@@ -221,10 +250,9 @@ hipError_t fk20::build_programs(Programs &P) {
     index.resize(TW_INDEX_SIZE);
     for (int j = 0; j < 10; j++) {
         index[TW_SCALE + j] = P.scale_off[j] = (uint32_t)ops.size();
-        append_program(ops, T.inv2k_std[j], TW_WNAF);
         fs expect;
         fs_set(expect, T.inv2k_mont[j]);
-        if (!check_program(ops.data() + index[TW_SCALE + j], expect)) return hipErrorInvalidValue;
+        if (!append_program(ops, T.inv2k_std[j], TW_WNAF, expect)) return hipErrorInvalidValue;
     }
     hipError_t err = hipMalloc(&P.ops, ops.size() * 4);
     if (err == hipSuccess) err = hipMalloc(&P.index, index.size() * 4);
@@ -803,6 +831,30 @@ int fk20_twiddle_selfcheck(unsigned *total_ops, unsigned *wnaf_width) {
     if (total_ops) *total_ops = (unsigned)ops.size();
     if (wnaf_width) *wnaf_width = (unsigned)TW_WNAF;
     return ok ? FK20_OK : FK20_EINVAL;
+}
+
+int fk20_twiddle_program(int which, unsigned idx, uint32_t *ops, unsigned max_ops, unsigned *n_ops) {
+    if (!n_ops || which < 0 || which > 2 || (which < 2 && idx >= 512) || (which == 2 && idx >= 10))
+        return FK20_EINVAL;
+    DevTables T;
+    build_host_tables(T);
+    const uint32_t *k;
+    const uint32_t *km;
+    if (which == 2) {
+        k = T.inv2k_std[idx];
+        km = T.inv2k_mont[idx];
+    } else {
+        const unsigned e = which == TW_FWD ? idx : (512 - idx) % 512;
+        k = T.tw_std[e];
+        km = T.tw_mont[e];
+    }
+    fs expect;
+    fs_set(expect, km);
+    std::vector<uint32_t> prog;
+    if (!append_program(prog, k, TW_WNAF, expect)) return FK20_EINVAL;
+    *n_ops = (unsigned)prog.size();
+    if (ops && max_ops) memcpy(ops, prog.data(), 4 * std::min<size_t>(max_ops, prog.size()));
+    return FK20_OK;
 }
 
 }  // extern "C"

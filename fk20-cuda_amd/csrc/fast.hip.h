@@ -1,32 +1,12 @@
 // Batched FK20 path: twiddle-program encoding and launchers (fast_kernels.hip, ctx.cpp).
 #pragma once
 #include "common.hip.h"
+#include "ladder.hip.h"
+#include "schedule.hip.h"
 
 namespace fk20 {
 
-// A twiddle multiplication x <- w^idx * x is a straight-line program over the wNAF digits of
-// a GLV split w^idx lambda^-t = k1 + k2 * lambda (|k1|, k2 < 2^128, phi(P) = (beta x, y) =
-// lambda P, t in {0, 1, 2}): k1's digits act on phi^t(P), k2's on phi^(t+1)(P).  One 32-bit
-// op per non-zero digit, read wave-uniformly:
-//   bits 0..7   doublings to perform before this op (for the END op: trailing doublings)
-//   bit  8      apply phi to the table entry (x times beta)
-//   bit  13     apply phi^2 instead (x times beta^2)
-//   bits 9..11  table entry e: the odd multiple (2e + 1) P
-//   bit  12     negate
-//   bit  31     END
-static constexpr uint32_t TWOP_DBL_MASK = 0xffu;
-static constexpr uint32_t TWOP_PHI = 1u << 8;
-static constexpr unsigned TWOP_ENTRY_SHIFT = 9;
-static constexpr uint32_t TWOP_NEG = 1u << 12;
-static constexpr uint32_t TWOP_PHI2 = 1u << 13;
-static constexpr uint32_t TWOP_END = 1u << 31;
-
-// wNAF width used by the G1 transform kernels (odd multiples P, 3P, .., 15P, kept in private
-// memory); the twiddle programs are built for the same width
-#ifndef FK20_TW_WNAF
-#define FK20_TW_WNAF 5
-#endif
-static constexpr int TW_WNAF = FK20_TW_WNAF;
+// Twiddle programs: op encoding and the ladder that runs them in ladder.hip.h.
 // direction rows of the program index: index[dir * 512 + idx] = offset of w^(+-idx)'s program
 static constexpr int TW_FWD = 0;
 static constexpr int TW_INV = 1;

@@ -381,54 +381,6 @@ __global__ void __launch_bounds__(64) k_commit_reduce(const uint32_t *in, uint32
     }
 }
 
-// ---------------------------------------------------------------- twiddle programs
-
-// x <- k * x for the wave-uniform scalar whose GLV + wNAF program starts at ops[0]
-// (op layout in fast.hip.h).  Table of odd multiples P, 3P, .., (2^(W-1)-1) P in registers.
-template <int W>
-__device__ __forceinline__ void tw_mul(g1p &x, const uint32_t *ops) {
-    constexpr int NT = 1 << (W - 2);
-    // The odd multiples live in private (scratch) memory, not registers: each addition
-    // reads one entry, chosen by a wave-uniform index, so the loads are coalesced per
-    // dword and the ladder's register footprint stays within two waves per SIMD.
-    g1p T[NT];
-    T[0] = x;
-    if (NT > 1) {
-        g1p x2, c = x;
-        g1_dbl(x2, x);
-#pragma unroll
-        for (int k = 1; k < NT; k++) {
-            g1_add(c, c, x2);
-            T[k] = c;
-        }
-    }
-    fe beta, beta2;
-    fe_set(beta, FP_BETA);
-    fe_set(beta2, FP_BETA2);
-    g1p acc;
-    bool started = false;
-#pragma unroll 1
-    for (unsigned k = 0;; k++) {
-        const uint32_t op = __builtin_amdgcn_readfirstlane(ops[k]);  // wave-uniform program
-        const unsigned nd = op & TWOP_DBL_MASK;
-#pragma unroll 1
-        for (unsigned d = 0; d < nd; d++) g1_dbl(acc, acc);
-        if (op & TWOP_END) break;
-        const unsigned e = (op >> TWOP_ENTRY_SHIFT) & (NT - 1);
-        g1p q = T[e];
-        if (op & TWOP_PHI2) fe_mul(q.x, q.x, beta2);
-        else if (op & TWOP_PHI) fe_mul(q.x, q.x, beta);
-        if (op & TWOP_NEG) fe_neg<16>(q.y, q.y);
-        if (started) {
-            g1_add(acc, acc, q);
-        } else {
-            acc = q;
-            started = true;
-        }
-    }
-    x = acc;
-}
-
 // ---------------------------------------------------------------- G1 transform stages
 //
 // Per item with L = log2(n2), stage schedule over the 2*Lmax - 1 launches of a batch:
@@ -439,92 +391,6 @@ __device__ __forceinline__ void tw_mul(g1p &x, const uint32_t *ops) {
 // item, local stage u = 0 .. 2L-2 numbers its own stages: DIT s = u (u < L-1), the middle
 // (u = L-1), DIF s = u - (L-1).  Butterfly b of a stage reads and writes two positions
 // (in place), given by stage_positions().
-
-struct Butterfly {
-    unsigned p0, p1;  // positions read and written
-    unsigned idx;     // twiddle exponent (w^+-idx), 0: none
-};
-
-// R22 = false: radix 2, every butterfly carries its stage's twiddle (the persistent chain).
-// R22 = true: radix 2^2 (the per-stage launches).  With the FFT's DIF stages numbered
-// s = 0 .. L-1 (the IFFT's DIT stage u is the transpose of DIF stage L-1-u), stages pair up
-// as (0, 1), (2, 3), ..: the even stage of a pair keeps only the w^(n/4) part (w^+-128 here,
-// a 64-bit GLV program) of its twiddle, on the half of its butterflies whose offset in the
-// block is at least half the span, and defers the rest to the odd stage, whose butterflies
-// carry no twiddle; a separate point-wise layer (k_twiddle_multi, layer_twiddle) then applies
-// the deferred and own twiddles of the odd stage, three per four points of each block.  A
-// lone last DIF stage (L odd) has only trivial twiddles.  Both schedules compute the same
-// transform exactly.
-// A radix-2^2 stage that carries w^+-128 (on the butterflies whose offset j in a block of
-// span 2^lh is at least 2^(lh-1)) numbers its butterflies as the middle does: the untwiddled
-// ones first, the twiddled ones in the second half of the launch (b -> block and offset with
-// the offset's top bit taken from b's top bit).  Measured on MI355X at 512 rows: the IFFT's
-// w^-128 launches 1.1 -> 0.67 ms (blocks in launch order gave some SIMDs two ladders and
-// others none); the FFT's w^128 launches, which follow a twiddle layer, stayed at 1.1 ms.
-__device__ __forceinline__ unsigned quarter_order(unsigned b, unsigned logn2, unsigned lh, unsigned &j) {
-    const unsigned c = b >> (logn2 - 2), r = b & ((1u << (logn2 - 2)) - 1);
-    j = (r & ((1u << (lh - 1)) - 1)) + (c << (lh - 1));
-    return r >> (lh - 1);  // block
-}
-
-template <bool R22>
-__device__ __forceinline__ Butterfly stage_positions(unsigned u, unsigned logn2, unsigned b) {
-    Butterfly f;
-    if (u + 1 < logn2) {  // IFFT DIT stage s = u: span 2^s
-        const unsigned s = u;
-        unsigned j = b & ((1u << s) - 1), blk = b >> s;
-        if (R22) {
-            const unsigned sd = logn2 - 1 - s;  // the transposed DIF stage
-            const bool quarter = (sd & 1) == 0 && sd + 1 < logn2;
-            if (quarter) blk = quarter_order(b, logn2, s, j);
-            f.idx = (quarter && 2 * j >= (1u << s)) ? 128u : 0u;
-        } else {
-            f.idx = j << (8 - s);
-        }
-        f.p0 = (blk << (s + 1)) + j;
-        f.p1 = f.p0 + (1u << s);
-    } else if (u + 1 == logn2) {  // middle: positions j, j + n2/2
-        f.p0 = b;
-        f.p1 = b + (1u << (logn2 - 1));
-        f.idx = R22 ? (4 * b >= (1u << logn2) ? 128u : 0u) : b << (9 - logn2);
-    } else {  // FFT DIF stage s = u - (L-1): span 2^(L-s-1)
-        const unsigned s = u + 1 - logn2, lh = logn2 - s - 1;
-        unsigned j = b & ((1u << lh) - 1), blk = b >> lh;
-        if (R22) {
-            const bool quarter = (s & 1) == 0 && s + 1 < logn2;
-            if (quarter) blk = quarter_order(b, logn2, lh, j);
-            f.idx = (quarter && 2 * j >= (1u << lh)) ? 128u : 0u;
-        } else {
-            f.idx = j << (8 - lh);
-        }
-        f.p0 = (blk << (lh + 1)) + j;
-        f.p1 = f.p0 + (1u << lh);
-    }
-    return f;
-}
-
-// The point-wise twiddles of odd DIF stage sd (span h = 2^(L-1-sd) >= 2) in the radix-2^2
-// schedule: in every block of 4h positions, quarter c (0..3) at offset j1 (0 < j1 < h) is
-// multiplied by w^(m_c j1 2^(sd-1)), m = (0, 2, 1, 3), in units of w of order 2^L (w512^(e)
-// with e scaled by 2^(9-L)).  Nontrivial twiddles are numbered k = 0 .. layer_count - 1.
-struct LayerTw {
-    unsigned q, e;  // position, exponent of w512
-};
-
-__host__ __device__ __forceinline__ unsigned layer_count(unsigned sd, unsigned logn2) {
-    const unsigned h = 1u << (logn2 - 1 - sd);
-    return h < 2 ? 0 : 3 * (h - 1) * ((1u << logn2) / (4 * h));
-}
-
-__device__ __forceinline__ LayerTw layer_twiddle(unsigned sd, unsigned logn2, unsigned k) {
-    const unsigned lh = logn2 - 1 - sd, h = 1u << lh, per = 3 * (h - 1);
-    const unsigned blk = k / per, r = k - blk * per;
-    const unsigned c = 1 + r / (h - 1), j1 = 1 + r % (h - 1);
-    LayerTw t;
-    t.q = (blk << (lh + 2)) + (c << lh) + j1;
-    t.e = ((c == 1 ? 2u : c == 2 ? 1u : 3u) * j1) << (sd - 1 + 9 - logn2);
-    return t;
-}
 
 // IFFT decimation in time (bit-reversed input): x = D[p1] w^-idx; D[p0], D[p1] = y + x, y - x.
 template <int W>

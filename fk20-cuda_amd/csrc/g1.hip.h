@@ -113,6 +113,114 @@ FK20_HD void g1_dbl(g1p &r, const g1p &p) {
     r.z = Z3;
 }
 
+// ---------------------------------------------------------------- Jacobian (ladders)
+//
+// Jacobian coordinates (x = X/Z^2, y = Y/Z^3) for the fixed-scalar ladders of the G1
+// transforms.  These formulas are incomplete: callers guarantee that no operand is infinity
+// and that no addition sees equal or opposite points (the twiddle programs are checked for
+// that on the host; see ladder.hip.h).  None of them uses the curve constant b, so they also
+// run on the isomorphic curve y^2 = x^3 + b u^6 on which a co-Z table's points are affine.
+struct g1j {
+    fe x, y, z;
+};
+
+// dbl-2009-l (Bernstein-Lange EFD, a = 0) with D = 4 X Y^2 from one product and
+// Y3 = E (D - X3) - 8 Y^4 as one shared-reduction sum of products: 3S + 2M + 1 fe_mul2.
+// In: X < 32 p, Y < 4 p, Z < 8 p.  Out: X < 18 p, Y < 2 p, Z < 4 p.
+FK20_HD void g1j_dbl(g1j &r, const g1j &p) {
+    fe A, B, D, E, F, t, u, Z3;
+    fe_sqr(A, p.x);                 // X^2 < 2p
+    fe_sqr(B, p.y);                 // Y^2 < 2p
+    fe_mul(D, p.x, B);              // < 2p
+    fe_mul_small<4>(D, D);          // D = 4 X Y^2 < 8p
+    fe_mul(Z3, p.y, p.z);           // 4*8 = 32p^2 -> < 2p
+    fe_add(Z3, Z3, Z3);             // 2 Y Z < 4p
+    fe_mul_small<3>(E, A);          // E = 3 X^2 < 6p
+    fe_sqr(F, E);                   // < 2p
+    fe_add(t, D, D);                // 2D < 16p
+    fe_sub<16>(r.x, F, t);          // X3 = F - 2D < 18p
+    fe_sub<32>(t, D, r.x);          // D - X3 < 40p
+    fe_mul_small<8>(u, B);          // 8 Y^2 < 16p
+    fe_neg<16>(u, u);               // -8 Y^2 < 16p
+    fe_mul2(r.y, E, t, B, u);       // E (D - X3) - 8 Y^4: 6*40 + 2*16 = 272p^2 -> < 2p
+    r.z = Z3;
+}
+
+// madd-2007-bl: r = p + (x2, y2), the affine operand on the same (possibly scaled) curve,
+// neither operand infinity, (x2, y2) != +-p.  4S + 5M + 1 fe_mul2.
+// In: X < 32 p, Y < 4 p, Z < 8 p; x2 < 6 p, y2 < 4 p.  Out: X < 10 p, Y < 2 p, Z < 6 p.
+FK20_HD void g1j_madd(g1j &r, const g1j &p, const fe &x2, const fe &y2) {
+    fe Z1Z1, U2, S2, H, HH, I, J, R, V, t;
+    fe_sqr(Z1Z1, p.z);              // < 2p
+    fe_mul(U2, x2, Z1Z1);           // < 2p
+    fe_mul(S2, p.z, Z1Z1);          // Z^3 < 2p
+    fe_mul(S2, y2, S2);             // < 2p
+    fe_sub<32>(H, U2, p.x);         // H = U2 - X1 < 34p
+    fe_sqr(HH, H);                  // 34^2 = 1156p^2 -> < 2p
+    fe_mul_small<4>(I, HH);         // I = 4 H^2 < 8p
+    fe_mul(J, H, I);                // 272p^2 -> < 2p
+    fe_sub<4>(R, S2, p.y);          // < 6p
+    fe_add(R, R, R);                // R = 2 (S2 - Y1) < 12p
+    fe_mul(V, p.x, I);              // 256p^2 -> < 2p
+    fe_add(t, V, V);                // 2V < 4p
+    fe_add(r.z, p.z, H);            // Z1 + H < 42p
+    fe_sqr(r.z, r.z);               // 1764p^2 -> < 2p
+    fe_sub2<4>(r.z, r.z, Z1Z1, HH); // Z3 = (Z1 + H)^2 - Z1Z1 - HH < 6p
+    fe sy;
+    fe_sqr(sy, R);                  // 144p^2 -> < 2p
+    fe_sub2<8>(sy, sy, J, t);       // X3 = R^2 - J - 2V < 10p
+    fe_sub<16>(V, V, sy);           // V - X3 < 18p
+    fe_add(J, J, J);                // 2J < 4p
+    fe_neg<4>(J, J);                // -2J < 4p
+    fe_mul2(r.y, R, V, p.y, J);     // R (V - X3) - 2 Y1 J: 12*18 + 4*4 = 232p^2 -> < 2p
+    r.x = sy;
+}
+
+// Doubling with update (the co-Z table start): q = 2p and (x1, y1) = p, both with
+// Z = 2 Y Z.  4S + 3M.  In: X < 32 p, Y < 4 p, Z < 10 p.  Out: q.x, q.y < 18 p, q.z < 4 p;
+// x1 < 8 p, y1 < 16 p.
+FK20_HD void g1j_dblu(g1j &q, fe &x1, fe &y1, const g1j &p) {
+    fe A, B, C, D, E, F, t;
+    fe_sqr(A, p.x);                 // < 2p
+    fe_sqr(B, p.y);                 // < 2p
+    fe_sqr(C, B);                   // Y^4 < 2p
+    fe_mul(D, p.x, B);              // < 2p
+    fe_mul_small<4>(D, D);          // D = 4 X Y^2 = X (2Y)^2 < 8p
+    fe_mul(q.z, p.y, p.z);          // 40p^2 -> < 2p
+    fe_add(q.z, q.z, q.z);          // 2 Y Z < 4p
+    fe_mul_small<3>(E, A);          // < 6p
+    fe_sqr(F, E);                   // < 2p
+    fe_add(t, D, D);                // < 16p
+    fe_sub<16>(q.x, F, t);          // X3 < 18p
+    fe_sub<32>(t, D, q.x);          // < 40p
+    fe_mul(q.y, E, t);              // < 2p
+    fe_mul_small<8>(C, C);          // 8 Y^4 = Y (2Y)^3 < 16p
+    fe_sub<16>(q.y, q.y, C);        // Y3 < 18p
+    x1 = D;
+    y1 = C;
+}
+
+// Co-Z addition with update (Meloni; Goundar-Joye-Miyaji "ZADDU"): for p1 = (x1, y1) and
+// p2 = (x2, y2) sharing Z, (x3, y3) = p1 + p2 and (x1, y1) <- p1, both with Z' = Z mu,
+// mu = x1 - x2 (returned).  p1 != +-p2.  2S + 4M.
+// In: coordinates < 18 p.  Out: x1, y1 < 2 p; x3 < 6 p, y3 < 4 p; mu < 50 p.
+FK20_HD void g1j_zaddu(fe &x3, fe &y3, fe &mu, fe &x1, fe &y1, const fe &x2, const fe &y2) {
+    fe C, W1, W2, D, dy, t;
+    fe_sub<32>(mu, x1, x2);         // < 50p
+    fe_sqr(C, mu);                  // 2500p^2 -> < 2p
+    fe_mul(W1, x1, C);              // < 2p
+    fe_mul(W2, x2, C);              // < 2p
+    fe_sub<32>(dy, y1, y2);         // < 50p
+    fe_sqr(D, dy);                  // < 2p
+    fe_sub<2>(t, W1, W2);           // < 4p
+    fe_mul(y1, y1, t);              // A1 = Y1 (W1 - W2) < 2p
+    fe_sub2<4>(x3, D, W1, W2);      // X3 = D - W1 - W2 < 6p
+    fe_sub<8>(t, W1, x3);           // < 10p
+    fe_mul(t, dy, t);               // 500p^2 -> < 2p
+    fe_sub<2>(y3, t, y1);           // Y3 = dy (W1 - X3) - A1 < 4p
+    x1 = W1;
+}
+
 // r = -p.  Input < 10 p (y bound), output y < 16 p... kept below 10p by using 16p - y
 // only where y < 16p; callers pass values < 4p or < 10p.
 FK20_HD void g1_neg(g1p &r, const g1p &p) {

@@ -190,6 +190,20 @@ def twiddle_selfcheck():
     return status, n.value, w.value
 
 
+def twiddle_program(which, idx):
+    """The ops (uint32 array) of the program the transform kernels run for w^idx (which 0),
+    w^-idx (which 1) or 2^-idx (which 2); host-only, no device needed."""
+    n = ctypes.c_uint()
+    status = lib().fk20_twiddle_program(which, idx, None, 0, ctypes.byref(n))
+    if status:
+        raise ValueError("fk20_twiddle_program(%d, %d): status %d" % (which, idx, status))
+    ops = np.zeros(n.value, dtype=np.uint32)
+    status = lib().fk20_twiddle_program(which, idx, ops.ctypes.data_as(ctypes.c_void_p), n.value, ctypes.byref(n))
+    if status:
+        raise ValueError("fk20_twiddle_program(%d, %d): status %d" % (which, idx, status))
+    return ops
+
+
 # fk20_ctx_set_output formats (include/fk20.h FK20_OUT_*)
 OUTPUT_FORMATS = {"g1p": 0, "g1p_da": 1, "compressed": 2, "compressed_da": 3}
 

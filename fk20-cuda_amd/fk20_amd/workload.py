@@ -94,114 +94,75 @@ def launch_mul32(N, kind, rows):
 # ---------------------------------------------------------------- executed-work model
 #
 # Fp multiplications the batched path's algorithms actually perform (same 300-mul32
-# convention), for the roofline of its kernels:
-#   twiddle w^e: GLV split w^e = k2*lambda + k1 (the cheapest of k or r - k, each with
-#     0 <= k1 < lambda or k1 - lambda, k2 + 1), wNAF-5 digits of k1 and k2; table
-#     P, 3P, .., 15P (1 dbl + 7 add); one dbl per digit position after the first, one add
-#     per non-zero digit after the first, one Fp mul per phi-digit
-#   butterfly: twiddle (if e != 0) + 2 complete additions
+# convention; a square and each product of a shared-reduction pair count as one):
+#   twiddle w^e: the library's own program (fk20_twiddle_program: GLV split, wNAF-5 digits)
+#     run by the co-Z Jacobian ladder -- table P, 3P, .., 15P with one common Z (conversion 3,
+#     doubling with update 7, seven co-Z additions 6 each, rescaling 34, table Z 1), one
+#     Jacobian doubling (7) per doubling op, one mixed addition (11) per digit after the first,
+#     one Fp mul per phi-digit, conversion back 4 and the infinity test 1
+#   butterfly: twiddle (if e != 0) + 2 complete additions (12 each)
 #   MSM: one mixed addition (11 Fp mul) per non-zero signed window digit, 16 terms per column
 
 R_MOD = 0x73EDA753299D7D483339D80809A1D80553BDA402FFFE5BFEFFFFFFFF00000001
-GLV_LAMBDA = 0xAC45A4010001A40200000000FFFFFFFF
 MIXED_ADD = 11
+J_DBL, J_MADD = 7, 11
 WNAF_W = 5  # the library reports its width (fk20_twiddle_selfcheck); a test ties the two
+_NT = 1 << (WNAF_W - 2)
+LADDER_FIXED = 3 + 7 + 6 * (_NT - 1) + 4 * (_NT - 1) + (_NT - 2) + 1 + 4 + 1
+
+TWOP_DBL_MASK, TWOP_PHI, TWOP_PHI2, TWOP_END = 0xFF, 1 << 8, 1 << 13, 1 << 31
+
+_programs = {}
 
 
-def _wnaf(k, w):
-    d = []
-    while k:
-        if k & 1:
-            x = k & ((1 << w) - 1)
-            if x >= 1 << (w - 1):
-                x -= 1 << w
-            k -= x
-        else:
-            x = 0
-        d.append(x)
-        k >>= 1
-    return d
-
-
-def _program_cost(d1, d2):
-    L = max(len(d1), len(d2))
-    nz = sum(1 for x in d1 if x) + sum(1 for x in d2 if x)
-    return max(L - 1, 0) * 8 + max(nz - 1, 0) * 12
-
-
-def glv_digits(k):
-    """The digit strings (k1, k2) and the phi power t of the library's program for k * P: of
-    the splits of k lambda^-t (t = 0, 1, 2; k1's digits then act on phi^t(P), k2's on
-    phi^(t+1)(P)) and of r minus it (then negated), each with 0 <= k1 < lambda or k1 - lambda
-    and k2 + 1, the first of least ladder cost (8 per doubling, 12 per addition)."""
-    best = None
-    lam_inv = pow(GLV_LAMBDA, 2, R_MOD)  # lambda^3 = 1
-    for t in range(3):
-        kt = k * pow(lam_inv, t, R_MOD) % R_MOD
-        for neg in (False, True):
-            if neg and kt == 0:
-                continue
-            kk = (R_MOD - kt) if neg else kt
-            k2, k1 = divmod(kk, GLV_LAMBDA)
-            for shifted in (False, True):
-                if shifted and k1 == 0:
-                    continue
-                d1 = _wnaf(GLV_LAMBDA - k1 if shifted else k1, WNAF_W)
-                d2 = _wnaf(k2 + 1 if shifted else k2, WNAF_W)
-                if shifted:
-                    d1 = [-x for x in d1]
-                if neg:
-                    d1, d2 = [-x for x in d1], [-x for x in d2]
-                c = _program_cost(d1, d2)
-                if best is None or c < best[0]:
-                    best = (c, d1, d2, t)
-    return best[1], best[2], best[3]
+def program_counts(ops):
+    """(doublings, additions, phi_muls, ops) of a program in the library's op encoding."""
+    ops = [int(x) for x in ops]
+    dbl = sum(op & TWOP_DBL_MASK for op in ops)
+    digits = [op for op in ops if not op & TWOP_END]
+    phi = sum(1 for op in digits if op & (TWOP_PHI | TWOP_PHI2))
+    return dbl, max(len(digits) - 1, 0), phi, len(ops)
 
 
 def twiddle_program(e_idx, inverse=False):
-    """(doublings, additions, phi_muls, ops) of the program for w^e (w of order 512)."""
-    w = pow(5, (R_MOD - 1) // 512, R_MOD)
-    k = pow(w, (512 - e_idx) % 512 if inverse else e_idx, R_MOD)
-    d1, d2, tphi = glv_digits(k)
-    L = max(len(d1), len(d2))
-    dbl = adds = phi = ops = 0
-    started = False
-    for t in range(L - 1, -1, -1):
-        if started:
-            dbl += 1
-        for part, d in ((0, d1), (1, d2)):
-            x = d[t] if t < len(d) else 0
-            if x:
-                ops += 1
-                phi += (tphi + part) % 3 != 0  # one Fp mul (beta or beta^2) per such digit
-                if started:
-                    adds += 1
-                started = True
-    return dbl, adds, phi, ops + 1
+    """(doublings, additions, phi_muls, ops) of the library's program for w^e (w of order 512)."""
+    key = (1 if inverse else 0, e_idx)
+    if key not in _programs:
+        from . import prover
+        _programs[key] = program_counts(prover.twiddle_program(*key))
+    return _programs[key]
 
 
 def twiddle_fp_muls(e_idx, inverse=False):
     dbl, adds, phi, _ = twiddle_program(e_idx, inverse)
-    table = DBL + ((1 << (WNAF_W - 2)) - 1) * ADD
-    return table + dbl * DBL + adds * ADD + phi
+    return LADDER_FIXED + dbl * J_DBL + adds * J_MADD + phi
+
+
+def _quarter_j(b, lg, lh):
+    """Offset j of launch index b in a renumbered w^+-128 stage of span 2^lh
+    (schedule.hip.h quarter_order: the untwiddled half of the launch first)."""
+    c, r = b >> (lg - 2), b & ((1 << (lg - 2)) - 1)
+    return (r & ((1 << (lh - 1)) - 1)) + (c << (lh - 1))
 
 
 def r22_butterfly_idx(u, lg, b):
-    """Twiddle exponent (units of w512) of butterfly b of local stage u in the radix-2^2
-    schedule of the per-stage launches (fast_kernels.hip stage_positions<true>): only the
+    """Twiddle exponent (units of w512) of launch index b of local stage u in the radix-2^2
+    schedule of the per-stage launches (schedule.hip.h stage_positions<true>): only the
     even DIF stages (and the transposed DIT stages) keep a w^(n/4) = w512^128 part."""
     n2 = 1 << lg
     if u + 1 < lg:  # DIT stage s = u
         s = u
-        j = b & ((1 << s) - 1)
         sd = lg - 1 - s
-        return 128 if (sd % 2 == 0 and sd + 1 < lg and 2 * j >= (1 << s)) else 0
+        if not (sd % 2 == 0 and sd + 1 < lg):
+            return 0
+        return 128 if 2 * _quarter_j(b, lg, s) >= (1 << s) else 0
     if u + 1 == lg:  # middle
         return 128 if 4 * b >= n2 else 0
     s = u + 1 - lg
     lh = lg - s - 1
-    j = b & ((1 << lh) - 1)
-    return 128 if (s % 2 == 0 and s + 1 < lg and 2 * j >= (1 << lh)) else 0
+    if not (s % 2 == 0 and s + 1 < lg):
+        return 0
+    return 128 if 2 * _quarter_j(b, lg, lh) >= (1 << lh) else 0
 
 
 def r22_layer(sd, lg):

@@ -255,6 +255,12 @@ int fk20_ctx_launch_times(fk20_ctx_t *ctx, float *ms, int *kinds, int max, int *
  * evaluated over Fr); total_ops receives the program size and wnaf_width the digit width
  * the programs (and the transform kernels' tables of odd multiples) use.  Needs no device. */
 int fk20_twiddle_selfcheck(unsigned *total_ops, unsigned *wnaf_width);
+/* Host-only copy of one program the transform kernels run: which = 0 for w^idx, 1 for
+ * w^-idx (idx < 512), 2 for 2^-idx (idx < 10).  Writes at most max_ops 32-bit ops (the
+ * encoding of ladder.hip.h) and the program's length to n_ops.  Needs no device.  No
+ * reference counterpart (the reference multiplies by twiddles with double-and-add,
+ * g1p_mul.cu:18). */
+int fk20_twiddle_program(int which, unsigned idx, uint32_t *ops, unsigned max_ops, unsigned *n_ops);
 
 /* Measurement helper: blocks x 256 lanes each issue 8 x iters dependent-free 32 x 32 -> 64-bit
  * multiply-adds (v_mad_u64_u32, the instruction every Fp product is built from); out receives

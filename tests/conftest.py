@@ -1,6 +1,8 @@
 """Shared pytest configuration: the `gpu` marker and golden-fixture loaders."""
+import ctypes
 import json
 import os
+import subprocess
 import sys
 
 import numpy as np
@@ -38,3 +40,20 @@ def hex_g1p(triple):
 @pytest.fixture(scope="session")
 def golden():
     return load_golden
+
+
+HOST_SRC = os.path.join(ROOT, "tests", "native", "arith_host.cpp")
+HOST_LIB = os.path.join(ROOT, "tests", "native", "_build", "libarith_host.so")
+
+
+@pytest.fixture(scope="session")
+def arith_host():
+    """The device arithmetic and index-map headers compiled by g++ (tests/native/arith_host.cpp),
+    rebuilt when a source is newer than the library."""
+    csrc = os.path.join(ROOT, "fk20-cuda_amd", "csrc")
+    deps = [HOST_SRC] + [os.path.join(csrc, f) for f in
+                         ("fp.hip.h", "fr.hip.h", "g1.hip.h", "ladder.hip.h", "schedule.hip.h", "constants.hip.h")]
+    if not os.path.exists(HOST_LIB) or os.path.getmtime(HOST_LIB) < max(os.path.getmtime(d) for d in deps):
+        os.makedirs(os.path.dirname(HOST_LIB), exist_ok=True)
+        subprocess.run(["g++", "-O2", "-std=c++17", "-fPIC", "-shared", "-o", HOST_LIB, HOST_SRC], check=True)
+    return ctypes.CDLL(HOST_LIB)

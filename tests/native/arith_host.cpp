@@ -4,7 +4,9 @@
 #define FK20_HD inline
 #define __host__
 #define __device__
-#include "../../fk20-cuda_amd/csrc/g1.hip.h"
+#define FK20_UNIFORM(x) (x)
+#include "../../fk20-cuda_amd/csrc/ladder.hip.h"
+#include "../../fk20-cuda_amd/csrc/schedule.hip.h"
 
 #include <cstring>
 
@@ -146,5 +148,59 @@ void host_g1_chain(uint64_t *r, const uint64_t *p, int steps) {
         g1_add(b, a, n);
     }
     stp(r, b);
+}
+
+// The transform kernels' fixed-scalar ladder (ladder.hip.h, table width FK20_TW_WNAF) on a
+// g1p_t, running the program `ops` (as fk20_twiddle_program returns it).
+void host_tw_mul(uint64_t *r, const uint64_t *p, const uint32_t *ops) {
+    g1p a;
+    ldp(a, p);
+    tw_mul<TW_WNAF>(a, ops);
+    stp(r, a);
+}
+
+// Jacobian doubling / mixed addition, g1p_t slots holding Jacobian (X, Y, Z) and the affine
+// operand (x, y) in the first two coordinates; `reps` repeats the operation on its output.
+void host_g1j_dbl(uint64_t *r, const uint64_t *p, int reps) {
+    g1j a;
+    ld(a.x, p);
+    ld(a.y, p + 6);
+    ld(a.z, p + 12);
+    for (int i = 0; i < reps; i++) g1j_dbl(a, a);
+    st(r, a.x);
+    st(r + 6, a.y);
+    st(r + 12, a.z);
+}
+
+void host_g1j_madd(uint64_t *r, const uint64_t *p, const uint64_t *q) {
+    g1j a;
+    fe x, y;
+    ld(a.x, p);
+    ld(a.y, p + 6);
+    ld(a.z, p + 12);
+    ld(x, q);
+    ld(y, q + 6);
+    g1j_madd(a, a, x, y);
+    st(r, a.x);
+    st(r + 6, a.y);
+    st(r + 12, a.z);
+}
+
+// The transform kernels' index maps (schedule.hip.h): butterfly b of local stage u as
+// (p0, p1, twiddle exponent) in the radix-2^2 (r22 = 1) or radix-2 schedule, and nontrivial
+// twiddle k of the layer of odd DIF stage sd as (position, exponent of w512).
+void host_stage_positions(int r22, unsigned u, unsigned logn2, unsigned b, unsigned *out) {
+    const Butterfly f = r22 ? stage_positions<true>(u, logn2, b) : stage_positions<false>(u, logn2, b);
+    out[0] = f.p0;
+    out[1] = f.p1;
+    out[2] = f.idx;
+}
+
+unsigned host_layer_count(unsigned sd, unsigned logn2) { return layer_count(sd, logn2); }
+
+void host_layer_twiddle(unsigned sd, unsigned logn2, unsigned k, unsigned *out) {
+    const LayerTw t = layer_twiddle(sd, logn2, k);
+    out[0] = t.q;
+    out[1] = t.e;
 }
 }

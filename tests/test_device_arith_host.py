@@ -3,8 +3,6 @@ from the same source the HIP kernels inline.  Exercises the lazy-reduction bound
 random and extreme inputs against Python integers and the oracle.
 """
 import ctypes
-import os
-import subprocess
 
 import numpy as np
 import pytest
@@ -12,21 +10,12 @@ import pytest
 import oracle as O
 from fk20_amd import synthetic as S
 
-HERE = os.path.dirname(os.path.abspath(__file__))
-SRC = os.path.join(HERE, "native", "arith_host.cpp")
-OUT = os.path.join(HERE, "native", "_build", "libarith_host.so")
 P, R = O.P_MOD, O.R_MOD
 
 
 @pytest.fixture(scope="module")
-def L():
-    deps = [SRC] + [os.path.join(HERE, "..", "fk20-cuda_amd", "csrc", f) for f in
-                    ("fp.hip.h", "fr.hip.h", "g1.hip.h", "constants.hip.h")]
-    if not os.path.exists(OUT) or os.path.getmtime(OUT) < max(os.path.getmtime(d) for d in deps):
-        os.makedirs(os.path.dirname(OUT), exist_ok=True)
-        subprocess.run(["g++", "-O2", "-std=c++17", "-fPIC", "-shared", "-o", OUT, SRC], check=True)
-    lib = ctypes.CDLL(OUT)
-    return lib
+def L(arith_host):
+    return arith_host
 
 
 def p64(a):
@@ -157,3 +146,91 @@ def test_g1_compress_against_oracle(L):
         out = np.zeros(48, dtype=np.uint8)
         L.host_g1_compress(out.ctypes.data_as(ctypes.c_void_p), p64(p))
         assert bytes(out) == bytes(O.g1_compress(p))
+
+
+# ---------------------------------------------------------------- Jacobian ladder
+
+def _affine_ints(p):
+    a = O.to_affine(np.ascontiguousarray(p, dtype=np.uint64))
+    return O.limbs_to_int(a[:6]), O.limbs_to_int(a[6:])
+
+
+def _jac(x, y, z):
+    """Jacobian slots (X, Y, Z) = (x z^2, y z^3, z) of the affine point (x, y), standard form."""
+    return np.concatenate([O.fp(x * z * z % P), O.fp(y * z * z * z % P), O.fp(z)])
+
+
+def _jac_affine(r):
+    X, Y, Z = (O.limbs_to_int(r[6 * i:6 * i + 6]) for i in range(3))
+    zi = pow(Z, P - 2, P)
+    return X * zi * zi % P, Y * zi * zi * zi % P
+
+
+def test_jacobian_dbl_and_madd(L):
+    """g1j_dbl (repeated, lazy bounds carried between steps) and g1j_madd against the oracle's
+    scalar multiples, from Jacobian representatives with random Z."""
+    G = O.g1p_gen()
+    rng = np.random.default_rng(41)
+    for a, b in [(1, 2), (5, 3), (R - 7, 11), (12345, R - 12344 - 5)] + [(int(x), int(y)) for x, y in
+                                                                       rng.integers(1, 2**62, (10, 2))]:
+        za = int.from_bytes(rng.bytes(48), "little") % P or 1
+        pa = _affine_ints(O.g1p_op("mul", G, O.fr(a)))
+        qb = _affine_ints(O.g1p_op("mul", G, O.fr(b)))
+        r = np.zeros(18, dtype=np.uint64)
+        reps = int(rng.integers(1, 40))
+        L.host_g1j_dbl(p64(r), p64(_jac(*pa, za)), reps)
+        assert _jac_affine(r) == _affine_ints(O.g1p_op("mul", G, O.fr(a * 2**reps % R)))
+        L.host_g1j_madd(p64(r), p64(_jac(*pa, za)), p64(np.concatenate([O.fp(qb[0]), O.fp(qb[1])])))
+        assert _jac_affine(r) == _affine_ints(O.g1p_op("mul", G, O.fr((a + b) % R)))
+
+
+def _programs():
+    from fk20_amd import prover
+    for which, count in ((0, 512), (1, 512), (2, 10)):
+        for idx in range(1 if which < 2 else 0, count):
+            yield which, idx, prover.twiddle_program(which, idx)
+
+
+def _program_scalar(which, idx):
+    w = pow(5, (R - 1) // 512, R)
+    if which == 2:
+        return pow(2, -idx, R)
+    return pow(w, idx if which == 0 else -idx, R)
+
+
+def test_ladder_runs_every_program(L):
+    """The transform kernels' ladder (co-Z table, Jacobian doublings, mixed additions; the
+    source the kernels inline) computes k P for every program the library builds -- w^+-i,
+    i = 1..511, and 2^-j -- on random points given as non-normalised projective inputs."""
+    rng = np.random.default_rng(42)
+    progs = list(_programs())
+    s = S.random_fr(len(progs), 43)
+    zs = np.array([O.fp(int.from_bytes(rng.bytes(48), "little") % (P - 1) + 1) for _ in progs])
+    pts = O.gen_mul_projective(s, zs)
+    ks = [O.fr(_program_scalar(w, i) * O.limbs_to_int(si) % R) for (w, i, _), si in zip(progs, s)]
+    exp = O.gen_mul_affine(np.array(ks))
+    out = np.zeros((len(progs), 18), dtype=np.uint64)
+    for n, (w, i, ops) in enumerate(progs):
+        L.host_tw_mul(p64(out[n]), p64(pts[n]), ops.ctypes.data_as(ctypes.c_void_p))
+    bad, first = O.match_affine(out, exp)
+    assert bad == 0, "program %s differs" % (progs[first][:2],)
+
+
+def test_ladder_infinity_and_structured_inputs(L):
+    """Infinity in any (0 : y : 0) form comes back as infinity; G, -G and small multiples (the
+    table entries' own scalars) give exact results."""
+    from fk20_amd import prover
+    G = O.g1p_gen()
+    for which, idx in ((0, 1), (1, 128), (0, 384), (2, 9)):
+        ops = prover.twiddle_program(which, idx)
+        k = _program_scalar(which, idx)
+        for y in (1, 7, P - 1):
+            inf = np.concatenate([O.fp(0), O.fp(y), O.fp(0)])
+            r = np.zeros(18, dtype=np.uint64)
+            L.host_tw_mul(p64(r), p64(inf), ops.ctypes.data_as(ctypes.c_void_p))
+            assert O.g1p_is_inf(r)
+        for m in (1, 2, 3, 15, R - 1, R - 15):
+            p = O.g1p_op("mul", G, O.fr(m))
+            r = np.zeros(18, dtype=np.uint64)
+            L.host_tw_mul(p64(r), p64(p), ops.ctypes.data_as(ctypes.c_void_p))
+            assert O.g1p_eq(r, O.g1p_op("mul", G, O.fr(k * m % R)))

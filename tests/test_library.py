@@ -82,12 +82,18 @@ def test_setup_table_bytes():
 
 
 def test_executed_work_model_matches_library_programs():
-    """The Python work model rebuilds the same twiddle programs the library runs."""
+    """The work model counts the library's own programs (fk20_twiddle_program), the same
+    ones the self-check builds, at the library's digit width."""
     from fk20_amd import prover, workload
     _, nops, width = prover.twiddle_selfcheck()
     assert workload.WNAF_W == width
     model = sum(workload.twiddle_program(e, inv)[3] for inv in (False, True) for e in range(512))
     assert model == nops
+    # w^128 = z lambda + z: 64-bit halves, about 62 doublings
+    dbl, adds, _, _ = workload.twiddle_program(128)
+    assert 55 <= dbl <= 66 and adds <= 30
+    for which, idx in ((0, 512), (1, 512), (2, 10), (3, 0), (-1, 0)):
+        assert L.lib().fk20_twiddle_program(which, idx, None, 0, ctypes.byref(ctypes.c_uint())) == -1
 
 
 def test_output_format_sizes_and_argument_checks():
