@@ -14,6 +14,24 @@
 
 using namespace fk20;
 
+// ---------------------------------------------------------------- static device memory
+//
+// The constant tables and twiddle programs of the drop-in entry points live in static device
+// memory (one copy per device, filled on the first call on that device), so no entry point
+// allocates them.
+
+static __device__ DevTables g_tables_dev;
+static constexpr size_t PROG_OPS_CAP = (size_t)1 << 16;  // words; the programs take about 48 K
+static __device__ uint32_t g_prog_ops[PROG_OPS_CAP];
+static __device__ uint32_t g_prog_index[TW_INDEX_SIZE];
+
+// Workspace of fk20_poly2h_fft_noalloc, the entry point for callers that cannot allocate:
+// a fixed array per device, held until the process ends.  Host code uses only the address
+// hipGetSymbolAddress returns for the current device (DeviceState::noalloc), never the
+// array's name.
+static constexpr size_t NOALLOC_SCRATCH_BYTES = FK20_NOALLOC_SCRATCH_BYTES;
+static __device__ __align__(256) unsigned char scratch[NOALLOC_SCRATCH_BYTES];
+
 namespace {
 
 thread_local int g_last_hip_error = 0;
@@ -39,10 +57,11 @@ unsigned log2u(unsigned n) { return 31u - (unsigned)__builtin_clz(n); }
 // ---------------------------------------------------------------- per-device state
 
 struct DeviceState {
-    DevTables *tables = nullptr;            // device copy
+    DevTables *tables = nullptr;            // device copy (g_tables_dev)
     Programs prog;                          // twiddle programs of the drop-in transforms
     void *scratch = nullptr;                // workspace of the drop-in entry points
     size_t scratch_bytes = 0;
+    unsigned char *noalloc = nullptr;       // this device's address of `scratch`
 };
 
 std::mutex g_mu;
@@ -111,19 +130,28 @@ int device_state(DeviceState **out) {
     if ((int)g_dev.size() <= dev) g_dev.resize(dev + 1);
     if (!g_dev[dev]) g_dev[dev].reset(new DeviceState());
     DeviceState &st = *g_dev[dev];
+    // set-up of this device, once, with the device current: symbol addresses of the static
+    // arrays, then the tables and programs copied into them (no allocation)
+    if (!st.noalloc) {
+        void *d = nullptr;
+        FK20_TRY(hipGetSymbolAddress(&d, HIP_SYMBOL(scratch)));
+        st.noalloc = reinterpret_cast<unsigned char *>(d);
+    }
     if (!st.tables) {
         DevTables h;
         build_host_tables(h);
         void *d = nullptr;
-        FK20_TRY(hipMalloc(&d, sizeof(DevTables)));
-        hipError_t e = hipMemcpy(d, &h, sizeof(DevTables), hipMemcpyHostToDevice);
-        if (e != hipSuccess) {
-            (void)hipFree(d);
-            return hip_status(e);
-        }
+        FK20_TRY(hipGetSymbolAddress(&d, HIP_SYMBOL(g_tables_dev)));
+        FK20_TRY(hipMemcpy(d, &h, sizeof(DevTables), hipMemcpyHostToDevice));
         st.tables = reinterpret_cast<DevTables *>(d);
     }
-    if (!st.prog.ops) FK20_TRY(build_programs(st.prog));
+    if (!st.prog.ops) {
+        void *ops = nullptr, *index = nullptr;
+        FK20_TRY(hipGetSymbolAddress(&ops, HIP_SYMBOL(g_prog_ops)));
+        FK20_TRY(hipGetSymbolAddress(&index, HIP_SYMBOL(g_prog_index)));
+        FK20_TRY(load_programs(st.prog, reinterpret_cast<uint32_t *>(ops), PROG_OPS_CAP,
+                               reinterpret_cast<uint32_t *>(index)));
+    }
     *out = &st;
     return FK20_OK;
 }
@@ -176,6 +204,7 @@ const char *fk20_status_string(int status) {
     case FK20_EHIP: return "HIP runtime error";
     case FK20_ENODEV: return "no gfx950 device";
     case FK20_ETIMEDOUT: return "a bounded device-side wait expired";
+    case FK20_ERANGE: return "request larger than the fixed workspace of an allocation-free entry point";
     default: return "unknown status";
     }
 }
@@ -282,20 +311,17 @@ int fk20_hext_fft2h_fft(fk20_g1p_t *h_fft, const fk20_g1p_t *hext_fft, unsigned 
 // fk20_poly2h_fft.cu:47-80: toeplitz -> NTT (16 per row) -> MSM -> IFFT -> zero -> FFT, with
 // the 1/n2 of the IFFT moved onto the NTT output (a field multiplication instead of a
 // scalar multiplication per point) and the transforms column-batched over the rows.
-int fk20_poly2h_fft(fk20_g1p_t *h_fft, const fk20_fr_t *poly, const fk20_g1p_t *xext_fft, unsigned rows,
-                    unsigned N, fk20_stream_t stream) {
-    if (!valid_N(N) || (rows && (!h_fft || !poly || !xext_fft))) return FK20_EINVAL;
-    if (!rows) return FK20_OK;
-    const unsigned n2 = N / 8;
-    DeviceState *st;
-    int rc = device_state(&st);
-    if (rc) return rc;
+// Workspace: tcT, D, the packed bases and the MSM digits, in that order.
+static size_t poly2h_scratch_bytes(unsigned rows, unsigned n2) {
+    return (size_t)rows * 16 * n2 * 32 + (size_t)rows * n2 * 144 + (size_t)16 * n2 * 144 +
+           msm_plain_digit_bytes(rows, n2);
+}
+
+static int poly2h_fft_in(void *scratch, DeviceState *st, fk20_g1p_t *h_fft, const fk20_fr_t *poly,
+                         const fk20_g1p_t *xext_fft, unsigned rows, unsigned n2, fk20_stream_t stream) {
     const size_t tc_bytes = (size_t)rows * 16 * n2 * 32;
     const size_t d_bytes = (size_t)rows * n2 * 144;
     const size_t x_bytes = (size_t)16 * n2 * 144;
-    void *scratch;
-    rc = device_scratch(st, tc_bytes + d_bytes + x_bytes + msm_plain_digit_bytes(rows, n2), &scratch);
-    if (rc) return rc;
     char *base = reinterpret_cast<char *>(scratch);
     uint32_t *tcT = reinterpret_cast<uint32_t *>(base);
     uint32_t *D = reinterpret_cast<uint32_t *>(base + tc_bytes);
@@ -305,6 +331,36 @@ int fk20_poly2h_fft(fk20_g1p_t *h_fft, const fk20_fr_t *poly, const fk20_g1p_t *
     FK20_TRY(launch_std_to_packed(xpk, W(xext_fft), (size_t)16 * n2, s));
     return hip_status(launch_plain(PLAIN_POLY2H_FFT, W(h_fft), W(poly), xpk, rows, log2u(n2), D, tcT, digits,
                                    st->tables, st->prog, s));
+}
+
+int fk20_poly2h_fft(fk20_g1p_t *h_fft, const fk20_fr_t *poly, const fk20_g1p_t *xext_fft, unsigned rows,
+                    unsigned N, fk20_stream_t stream) {
+    if (!valid_N(N) || (rows && (!h_fft || !poly || !xext_fft))) return FK20_EINVAL;
+    if (!rows) return FK20_OK;
+    const unsigned n2 = N / 8;
+    DeviceState *st;
+    int rc = device_state(&st);
+    if (rc) return rc;
+    void *scr;
+    rc = device_scratch(st, poly2h_scratch_bytes(rows, n2), &scr);
+    if (rc) return rc;
+    return poly2h_fft_in(scr, st, h_fft, poly, xext_fft, rows, n2, stream);
+}
+
+// The same launch sequence with its workspace in this device's fixed `scratch` array: no
+// allocation (the first call on a device copies the constant tables into static device
+// memory, which allocates nothing either).  Calls that overlap on one device race on the
+// array; nothing in it needs clearing (every byte is written before it is read).
+int fk20_poly2h_fft_noalloc(fk20_g1p_t *h_fft, const fk20_fr_t *poly, const fk20_g1p_t *xext_fft, unsigned rows,
+                            unsigned N, fk20_stream_t stream) {
+    if (!valid_N(N) || (rows && (!h_fft || !poly || !xext_fft))) return FK20_EINVAL;
+    if (!rows) return FK20_OK;
+    const unsigned n2 = N / 8;
+    if (poly2h_scratch_bytes(rows, n2) > NOALLOC_SCRATCH_BYTES) return FK20_ERANGE;
+    DeviceState *st;
+    int rc = device_state(&st);
+    if (rc) return rc;
+    return poly2h_fft_in(st->noalloc, st, h_fft, poly, xext_fft, rows, n2, stream);
 }
 
 int fk20_setup2xext_fft(fk20_g1p_t *xext_fft, const fk20_g1p_t *setup, unsigned N, fk20_stream_t stream) {

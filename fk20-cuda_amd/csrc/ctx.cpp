@@ -242,18 +242,27 @@ int status_of(hipError_t e) { return status_from_hip(e); }
 
 }  // namespace
 
-hipError_t fk20::build_programs(Programs &P) {
-    std::vector<uint32_t> ops, index;
-    if (!build_programs_host(ops, index)) return hipErrorInvalidValue;
+namespace {
+// Every program (w^+-i, then 2^-j) on the host, each checked, with its index table.
+bool all_programs_host(std::vector<uint32_t> &ops, std::vector<uint32_t> &index, uint32_t *scale_off) {
+    if (!build_programs_host(ops, index)) return false;
     DevTables T;
     build_host_tables(T);
     index.resize(TW_INDEX_SIZE);
     for (int j = 0; j < 10; j++) {
-        index[TW_SCALE + j] = P.scale_off[j] = (uint32_t)ops.size();
+        index[TW_SCALE + j] = scale_off[j] = (uint32_t)ops.size();
         fs expect;
         fs_set(expect, T.inv2k_mont[j]);
-        if (!append_program(ops, T.inv2k_std[j], TW_WNAF, expect)) return hipErrorInvalidValue;
+        if (!append_program(ops, T.inv2k_std[j], TW_WNAF, expect)) return false;
     }
+    return true;
+}
+}  // namespace
+
+hipError_t fk20::build_programs(Programs &P) {
+    std::vector<uint32_t> ops, index;
+    if (!all_programs_host(ops, index, P.scale_off)) return hipErrorInvalidValue;
+    P.owned = true;
     hipError_t err = hipMalloc(&P.ops, ops.size() * 4);
     if (err == hipSuccess) err = hipMalloc(&P.index, index.size() * 4);
     if (err == hipSuccess) err = hipMemcpy(P.ops, ops.data(), ops.size() * 4, hipMemcpyHostToDevice);
@@ -262,9 +271,24 @@ hipError_t fk20::build_programs(Programs &P) {
     return err;
 }
 
+hipError_t fk20::load_programs(Programs &P, uint32_t *ops_dev, size_t ops_cap, uint32_t *index_dev) {
+    std::vector<uint32_t> ops, index;
+    if (!all_programs_host(ops, index, P.scale_off) || ops.size() > ops_cap) return hipErrorInvalidValue;
+    hipError_t err = hipMemcpy(ops_dev, ops.data(), ops.size() * 4, hipMemcpyHostToDevice);
+    if (err == hipSuccess) err = hipMemcpy(index_dev, index.data(), index.size() * 4, hipMemcpyHostToDevice);
+    if (err == hipSuccess) {
+        P.ops = ops_dev;
+        P.index = index_dev;
+        P.owned = false;
+    }
+    return err;
+}
+
 void fk20::free_programs(Programs &P) {
-    if (P.ops) (void)hipFree(P.ops);
-    if (P.index) (void)hipFree(P.index);
+    if (P.owned) {
+        if (P.ops) (void)hipFree(P.ops);
+        if (P.index) (void)hipFree(P.index);
+    }
     P.ops = P.index = nullptr;
 }
 

@@ -20,10 +20,14 @@ struct Programs {
     uint32_t *ops = nullptr;
     uint32_t *index = nullptr;
     uint32_t scale_off[10] = {};
+    bool owned = false;  // ops and index were allocated by build_programs
 };
 // Builds and checks every program on the host (each evaluates to its scalar over Fr with
-// phi = lambda), then copies them to the current device.
+// phi = lambda), then copies them to new allocations on the current device.
 hipError_t build_programs(Programs &P);
+// The same, copied into caller-owned device memory (ops_cap words of ops, TW_INDEX_SIZE
+// words of index); no allocation.
+hipError_t load_programs(Programs &P, uint32_t *ops_dev, size_t ops_cap, uint32_t *index_dev);
 void free_programs(Programs &P);
 
 hipError_t launch_scale_bases(uint32_t *xs, uint32_t *inf, const uint32_t *xext, size_t count,

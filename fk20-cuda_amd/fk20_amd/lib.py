@@ -14,7 +14,8 @@ LIBPATH = os.environ.get("FK20_LIBRARY", os.path.join(PKG_DIR, "libfk20.so"))
 
 FK20_OK = 0
 STATUS = {0: "ok", -1: "invalid argument", -2: "out of device memory", -3: "HIP runtime error",
-          -4: "no gfx950 device", -5: "a bounded device-side wait expired"}
+          -4: "no gfx950 device", -5: "a bounded device-side wait expired",
+          -6: "request larger than the fixed workspace of an allocation-free entry point"}
 
 
 class Fk20Error(RuntimeError):
@@ -43,6 +44,7 @@ _SIGS = {
     "fk20_hext2h": ([_P, _U, _U, _P], _I),
     "fk20_hext_fft2h_fft": ([_P, _P, _U, _U, _P], _I),
     "fk20_poly2h_fft": ([_P, _P, _P, _U, _U, _P], _I),
+    "fk20_poly2h_fft_noalloc": ([_P, _P, _P, _U, _U, _P], _I),
     "fk20_setup2xext_fft": ([_P, _P, _U, _P], _I),
     "fk20_g1p_to_affine": ([_P, _P, _SZ, _P], _I),
     "fk20_twiddle_program": ([_I, _U, _P, _U, ctypes.POINTER(_U)], _I),

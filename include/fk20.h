@@ -43,7 +43,8 @@ enum {
     FK20_ENOMEM = -2,   /* device allocation failed or budget exceeded */
     FK20_EHIP = -3,     /* HIP runtime error (fk20_last_hip_error gives the code) */
     FK20_ENODEV = -4,   /* no usable gfx950 device */
-    FK20_ETIMEDOUT = -5 /* a bounded device-side wait expired; the outputs are not valid */
+    FK20_ETIMEDOUT = -5, /* a bounded device-side wait expired; the outputs are not valid */
+    FK20_ERANGE = -6     /* request larger than the fixed workspace of fk20_poly2h_fft_noalloc */
 };
 
 /* Last HIP error code seen by the library in this thread (0 if none). */
@@ -101,6 +102,21 @@ int fk20_hext_fft2h_fft(fk20_g1p_t *h_fft, const fk20_g1p_t *hext_fft, unsigned 
  * context API below for concurrent or allocation-free use. */
 int fk20_poly2h_fft(fk20_g1p_t *h_fft, const fk20_fr_t *polynomial, const fk20_g1p_t *xext_fft,
                     unsigned rows, unsigned N, fk20_stream_t stream);
+
+/* fk20_poly2h_fft without any allocation, for callers that cannot allocate device memory
+ * (code under stream capture, tight serving loops, hosts that forbid hipMalloc).  No
+ * reference counterpart: the reference's fk20_poly2h_fft owns static managed scratch
+ * (fk20_poly2h_fft.cu:13-16) but has no allocation-free contract.  The workspace is a fixed
+ * FK20_NOALLOC_SCRATCH_BYTES array in device memory, one per device the library runs on, held
+ * until the process ends; a call needs rows * (N/8) * 1344 + (N/8) * 2304 bytes of it, so
+ * rows <= 95 at N = 4096, 193 at N = 2048 and 388 at N = 1024, and larger requests return
+ * FK20_ERANGE without touching the device.  Every call on a device shares that device's
+ * array: calls that overlap on one device (different streams, threads) race and must be
+ * serialised by the caller.  The first call of any entry point on a device copies the
+ * library's constant tables into static device memory (no allocation there either). */
+#define FK20_NOALLOC_SCRATCH_BYTES ((size_t)64 << 20)
+int fk20_poly2h_fft_noalloc(fk20_g1p_t *h_fft, const fk20_fr_t *polynomial, const fk20_g1p_t *xext_fft,
+                            unsigned rows, unsigned N, fk20_stream_t stream);
 
 /* setup SRS [tau^i]G (i = 0..N) -> xext_fft[16][n2] by FK20Py toeplitz_part1.
  * Replaces fk20_setup2xext_fft (fk20.cuh:20; fk20.cu:28), whose reference version is

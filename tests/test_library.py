@@ -96,6 +96,19 @@ def test_executed_work_model_matches_library_programs():
         assert L.lib().fk20_twiddle_program(which, idx, None, 0, ctypes.byref(ctypes.c_uint())) == -1
 
 
+def test_noalloc_range_check_before_device():
+    """fk20_poly2h_fft_noalloc rejects a request larger than its fixed workspace with
+    FK20_ERANGE (and bad sizes with FK20_EINVAL) before any device call."""
+    f = L.lib().fk20_poly2h_fft_noalloc
+    p = ctypes.c_void_p(16)
+    assert f(p, p, p, 96, 4096, None) == -6
+    assert f(p, p, p, 194, 2048, None) == -6
+    assert f(p, p, p, 389, 1024, None) == -6
+    assert f(p, p, p, 1, 4095, None) == -1
+    assert f(None, p, p, 1, 4096, None) == -1
+    assert f(None, None, None, 0, 4096, None) == 0
+
+
 def test_output_format_sizes_and_argument_checks():
     lib = L.lib()
     assert [lib.fk20_output_point_bytes(f) for f in (0, 1, 2, 3, 4, -1)] == [144, 144, 48, 48, 0, 0]
