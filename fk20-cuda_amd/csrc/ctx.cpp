@@ -464,15 +464,16 @@ static hipError_t build_tables(Setup &st, const fk20_g1p_t *points, unsigned nba
                                hipStream_t s) {
     const unsigned H = 1u << (st.win - 1);
     uint32_t *xs = nullptr, *zt = nullptr;
-    // bases per build launch, bounding the batch-inversion scratch to ~2 GiB (one thread per
-    // (base, window) builds its 2^(w-1) entries, so larger chunks mean more parallel threads)
+    // bases per build launch: one thread per (base, window), batch-inversion scratch bounded
+    // by 4 GiB (every base of an N = 4096 setup in one launch up to w = 13)
     const size_t per_base = (size_t)st.nwin * H * 96;
-    unsigned chunk = (unsigned)std::max<size_t>(1, ((size_t)2 << 30) / per_base);
+    const size_t zt_per_base = table_build_scratch_bytes(1, st.win, st.nwin);
+    unsigned chunk = (unsigned)std::max<size_t>(1, ((size_t)4 << 30) / zt_per_base);
     if (chunk > nbases) chunk = nbases;
     hipError_t e = hipMalloc(&st.table, (size_t)nbases * per_base);
     if (e == hipSuccess) e = hipMalloc(&st.inf, nbases * 4);
     if (e == hipSuccess) e = hipMalloc(&xs, (size_t)nbases * 144);
-    if (e == hipSuccess) e = hipMalloc(&zt, (size_t)chunk * per_base);
+    if (e == hipSuccess) e = hipMalloc(&zt, table_build_scratch_bytes(chunk, st.win, st.nwin));
     if (e == hipSuccess)
         e = launch_scale_bases(xs, st.inf, reinterpret_cast<const uint32_t *>(points), nbases, scale, s);
     for (unsigned b0 = 0; e == hipSuccess && b0 < nbases; b0 += chunk)

@@ -32,6 +32,13 @@ void free_programs(Programs &P);
 
 hipError_t launch_scale_bases(uint32_t *xs, uint32_t *inf, const uint32_t *xext, size_t count,
                               const uint32_t *scale, hipStream_t s);
+// Window tables of bases [base0, base0 + nb): zt is workspace of table_build_scratch_bytes(nb,
+// win_bits, nwin) bytes.
+static constexpr unsigned TABLE_BLK = 256;  // entries per batch inversion
+inline size_t table_build_scratch_bytes(unsigned nb, unsigned win_bits, unsigned nwin) {
+    const unsigned H = 1u << (win_bits - 1);
+    return (size_t)nb * nwin * (H < TABLE_BLK ? H : TABLE_BLK) * 96;
+}
 hipError_t launch_table_build(uint32_t *table, uint32_t *zt, const uint32_t *xs, const uint32_t *inf,
                               unsigned base0, unsigned nb, unsigned win_bits, unsigned nwin, hipStream_t s);
 // KZG commitments of `rows` polynomials of length N over an SRS setup's window tables.

@@ -106,18 +106,21 @@ __global__ void __launch_bounds__(64) k_scale_bases(uint32_t *xs, uint32_t *inf,
 
 // ---------------------------------------------------------------- setup: window tables
 
-// One thread per (base, window) of the chunk [base0, base0 + nb).  Entries are built as
-// projective multiples d*B (d = 1..H) and converted to affine with one inversion per thread
-// (Montgomery's batch trick).  zt holds (z_d, prefix_{d-1}) per entry for the backward pass.
-__global__ void __launch_bounds__(64) k_table_build(uint32_t *table, uint32_t *zt, const uint32_t *xs,
+// One thread per (base, window) of the launch's bases [base0, base0 + nb): entries d * B' for
+// d = 1 .. H (B' = 2^(w win) B), built as projective multiples and converted to affine by
+// Montgomery's batch trick in blocks of TABLE_BLK entries (one inversion per block).  zt holds
+// (z_e, prefix_{e-1}) of the current block per thread, laid out [entry][thread] so that the
+// 64 lanes of a wave touch consecutive words; it is sized by the launch, not by H, so every
+// (base, window) of a setup fits in one or two launches.
+__global__ void __launch_bounds__(64, 3) k_table_build(uint32_t *table, uint32_t *zt, const uint32_t *xs,
                                                     const uint32_t *inf, unsigned base0, unsigned nb,
                                                     unsigned win_bits, unsigned nwin) {
+    const unsigned nthreads = nb * nwin;
     const unsigned t = blockIdx.x * 64 + threadIdx.x;
-    if (t >= nb * nwin) return;
+    if (t >= nthreads) return;
     const unsigned b = base0 + t / nwin, win = t % nwin;
-    const unsigned H = 1u << (win_bits - 1);
+    const unsigned H = 1u << (win_bits - 1), blk = min(H, TABLE_BLK);
     uint32_t *tab = table + ((size_t)b * nwin + win) * H * G1A_WORDS;
-    uint32_t *z = zt + (size_t)t * H * 24;
     if (inf[b]) {
         for (unsigned e = 0; e < H; e++) {
             uint32_t zero[24] = {0};
@@ -125,40 +128,56 @@ __global__ void __launch_bounds__(64) k_table_build(uint32_t *table, uint32_t *z
         }
         return;
     }
-    g1p B, E;
-    pt_load(B, xs + (size_t)b * 36);
+    // B' = 2^(w win) B in affine form, so each entry is one mixed addition of B' (B' is not
+    // infinity: the base is not, and its order r is odd)
+    g1p E;
+    fe bx, by;
+    {
+        g1p B;
+        pt_load(B, xs + (size_t)b * 36);
 #pragma unroll 1
-    for (unsigned d = 0; d < win_bits * win; d++) g1_dbl(B, B);
-    E = B;
-    fe prefix;
-    fe_one(prefix);
-#pragma unroll 1
-    for (unsigned e = 0; e < H; e++) {
-        if (e) g1_add(E, E, B);
-        fe_store(tab + (size_t)e * G1A_WORDS, E.x);
-        fe_store(tab + (size_t)e * G1A_WORDS + 12, E.y);
-        fe_store(z + (size_t)e * 24, E.z);
-        fe_store(z + (size_t)e * 24 + 12, prefix);  // prefix of z_0 .. z_{e-1}
-        fe_mul(prefix, prefix, E.z);
+        for (unsigned d = 0; d < win_bits * win; d++) g1_dbl(B, B);
+        g1_to_affine(bx, by, B);
     }
-    fe inv;
-    fe_canonical(prefix, prefix);
-    fe_inv(inv, prefix);
+    E.x = bx;
+    E.y = by;
+    fe_one(E.z);
 #pragma unroll 1
-    for (int e = (int)H - 1; e >= 0; e--) {
-        fe ze, pe, zinv, x, y;
-        fe_load(ze, z + (size_t)e * 24);
-        fe_load(pe, z + (size_t)e * 24 + 12);
-        fe_mul(zinv, inv, pe);
-        fe_mul(inv, inv, ze);
-        fe_load(x, tab + (size_t)e * G1A_WORDS);
-        fe_load(y, tab + (size_t)e * G1A_WORDS + 12);
-        fe_mul(x, x, zinv);
-        fe_mul(y, y, zinv);
-        fe_canonical(x, x);
-        fe_canonical(y, y);
-        fe_store(tab + (size_t)e * G1A_WORDS, x);
-        fe_store(tab + (size_t)e * G1A_WORDS + 12, y);
+    for (unsigned e0 = 0; e0 < H; e0 += blk) {
+        fe prefix;
+        fe_one(prefix);
+#pragma unroll 1
+        for (unsigned i = 0; i < blk; i++) {
+            const unsigned e = e0 + i;
+            if (e) g1_add_aff(E, E, bx, by);
+            uint32_t *z = zt + ((size_t)i * nthreads + t) * 24;
+            fe_store(tab + (size_t)e * G1A_WORDS, E.x);
+            fe_store(tab + (size_t)e * G1A_WORDS + 12, E.y);
+            fe_store(z, E.z);
+            fe_store(z + 12, prefix);  // prefix of z_e0 .. z_{e-1}
+            fe_mul(prefix, prefix, E.z);
+        }
+        fe inv;
+        fe_canonical(prefix, prefix);
+        fe_inv(inv, prefix);
+#pragma unroll 1
+        for (int i = (int)blk - 1; i >= 0; i--) {
+            const unsigned e = e0 + (unsigned)i;
+            const uint32_t *z = zt + ((size_t)i * nthreads + t) * 24;
+            fe ze, pe, zinv, x, y;
+            fe_load(ze, z);
+            fe_load(pe, z + 12);
+            fe_mul(zinv, inv, pe);
+            fe_mul(inv, inv, ze);
+            fe_load(x, tab + (size_t)e * G1A_WORDS);
+            fe_load(y, tab + (size_t)e * G1A_WORDS + 12);
+            fe_mul(x, x, zinv);
+            fe_mul(y, y, zinv);
+            fe_canonical(x, x);
+            fe_canonical(y, y);
+            fe_store(tab + (size_t)e * G1A_WORDS, x);
+            fe_store(tab + (size_t)e * G1A_WORDS + 12, y);
+        }
     }
 }
 
