@@ -247,11 +247,11 @@ def time_f1(b, prover, sid, reps):
 
 # ---------------------------------------------------------------- C4: uneven batch
 
-def time_uneven(prover, rows_each, window, steps, seed):
+def time_uneven(prover, rows_each, window, steps, seed, keep=False):
     """512 rows each of N = 1024 / 2048 / 4096 (three setups) in one fk20_prove_batch pass;
-    each setup uses its autotuned window (tuned.json) or `window`.  Returns the pass times and,
-    per group, (N, secret, host polynomials, host h_fft of the last timed pass) for the check
-    against the oracle in the CPU leg."""
+    each setup uses its autotuned window (tuned.json) or `window`.  Returns the pass times,
+    per group (N, secret, host polynomials, host h_fft of the last timed pass) for the check
+    against the oracle in the CPU leg, and (keep=True) the setups left loaded, by N."""
     from fk20_amd.autotune import tuned_window
 
     items, bufs, groups = [], [], []
@@ -279,14 +279,18 @@ def time_uneven(prover, rows_each, window, steps, seed):
     stream.synchronize()
     for g in groups:
         g[3] = g[3].download((rows_each, g[0] // 8, 18))
-    for sid, _, _, _ in items:
-        prover.unload_setup(sid)
+    kept = {}
+    for (sid, _, _, _), g in zip(items, groups):
+        if keep:
+            kept[g[0]] = sid
+        else:
+            prover.unload_setup(sid)
     for x in bufs:
         x.free()
     e0.destroy()
     e1.destroy()
     stream.destroy()
-    return ms, groups
+    return ms, groups, kept
 
 
 # ---------------------------------------------------------------- C5: serving loop
@@ -487,30 +491,27 @@ def bench_commit(args):
     stream.destroy()
 
 
-def bench_serve(args):
-    """Poisson arrivals of one-row blobs, N = 1024/2048/4096 weighted 1:2:5, offered load swept."""
+def serve_levels(prover, sids, rates, duration, budget_bytes, slots, max_batch_rows, seed, stop_past_saturation=True):
+    """Poisson arrivals of one-row blobs, N = 1024/2048/4096 weighted 1:2:5, into the serving
+    loop (fk20_server_*) over `prover`'s resident setups sids[N], at each offered load in
+    rows/s for `duration` seconds; latency = completion - submission of each request."""
     import threading
 
     from fk20_amd.lib import PinnedArray
     from fk20_amd.server import Server
 
-    prover = Prover()
-    sids, polys, outs = {}, {}, {}
+    polys, outs = {}, {}
     pool = 2048
     for k, N in enumerate((1024, 2048, 4096)):
-        xext = device_setup(N, S.random_secret(args.seed + 100 + k))
-        sids[N] = prover.load_setup(xext, N, args.window)
-        xext.free()
         polys[N] = PinnedArray((64, N, 4))
-        polys[N].array[:] = S.random_polys(64, N, args.seed + 300 + k)
+        polys[N].array[:] = S.random_polys(64, N, seed + 300 + k)
         outs[N] = PinnedArray((pool, N // 8, 18))
     sizes = np.array([1024] * 1 + [2048] * 2 + [4096] * 5)
-    rates = [float(r) for r in args.rates.split(",")]
     levels = []
     for rate in rates:
-        srv = Server(prover, budget_bytes=args.budget_gb << 30, slots=args.slots, max_batch_rows=args.max_batch_rows)
-        rng = np.random.default_rng(args.seed + int(rate))
-        n = max(64, int(rate * args.duration))
+        srv = Server(prover, budget_bytes=budget_bytes, slots=slots, max_batch_rows=max_batch_rows)
+        rng = np.random.default_rng(seed + int(rate))
+        n = max(64, int(rate * duration))
         gaps = rng.exponential(1.0 / rate, size=n)
         which = sizes[rng.integers(0, len(sizes), size=n)]
         ids = [None] * n
@@ -554,10 +555,32 @@ def bench_serve(args):
         })
         log("serve %.0f rows/s offered: sustained %.0f, p50 %.2f ms, p99 %.2f ms, %d batches"
             % (rate, levels[-1]["sustained_rows_per_s"], levels[-1]["p50_ms"], levels[-1]["p99_ms"], st["batches"]))
-        if levels[-1]["sustained_rows_per_s"] < 0.8 * rate:
+        if stop_past_saturation and levels[-1]["sustained_rows_per_s"] < 0.8 * rate:
             break  # past saturation
     for a in list(polys.values()) + list(outs.values()):
         a.free()
+    return levels
+
+
+def serving_setups(prover, window, seed):
+    """The three resident setups (N = 1024 / 2048 / 4096) of the serving workload."""
+    from fk20_amd.autotune import tuned_window
+
+    sids = {}
+    for k, N in enumerate((1024, 2048, 4096)):
+        xext = device_setup(N, S.random_secret(seed + 100 + k))
+        sids[N] = prover.load_setup(xext, N, tuned_window(N, window))
+        xext.free()
+    return sids
+
+
+def bench_serve(args):
+    """Poisson arrivals of one-row blobs, N = 1024/2048/4096 weighted 1:2:5, offered load swept."""
+    prover = Prover()
+    sids = serving_setups(prover, args.window, args.seed)
+    rates = [float(r) for r in args.rates.split(",")]
+    levels = serve_levels(prover, sids, rates, args.duration, args.budget_gb << 30, args.slots,
+                          args.max_batch_rows, args.seed)
     prover.close()
     best = max(levels, key=lambda x: x["sustained_rows_per_s"])
     out = {
@@ -672,6 +695,9 @@ def main():
     ap.add_argument("--rates", default="1000,2000,4000,6000,8000,10000,14000",
                     help="serve: offered loads in rows/s")
     ap.add_argument("--duration", type=float, default=2.0, help="serve: seconds per load level")
+    ap.add_argument("--serve-rates", default="2000,8000",
+                    help="batch workload: offered loads (rows/s) of the serving leg after the batch legs "
+                         "(empty: skip); the line's serving p50/p99 are those of the last load")
     ap.add_argument("--budget-gb", type=int, default=8, help="serve: device budget for in-flight rows")
     ap.add_argument("--slots", type=int, default=2, help="serve: batches in flight")
     ap.add_argument("--max-batch-rows", type=int, default=1024)
@@ -824,12 +850,31 @@ def main():
         forms["F_1row_latency_ms_p50"] = round(statistics.median(f1), 3)
     # the F output of the last timed step (the graph replays rewrote it)
     gpu_out = b.h_fft.download((b.rows, b.n2, 18))
-    uneven, uneven_groups = None, None
+    uneven, uneven_groups, serve_sids = None, None, {}
     if args.uneven_rows:
         log("timing uneven batch: %d rows each of N=1024/2048/4096 in one pass" % args.uneven_rows)
-        um, uneven_groups = time_uneven(prover, args.uneven_rows, args.window, max(2, args.steps // 2), args.seed)
+        um, uneven_groups, serve_sids = time_uneven(prover, args.uneven_rows, args.window,
+                                                    max(2, args.steps // 2), args.seed, keep=True)
         uneven = {"rows": 3 * args.uneven_rows, "ms_p50": round(statistics.median(um), 3),
                   "rows_per_s": round(3 * args.uneven_rows / (statistics.median(um) / 1e3), 1)}
+    serving = None
+    if args.serve_rates:
+        # the serving loop over the same three resident setups: one-row blobs, N = 1024 / 2048 /
+        # 4096 weighted 1:2:5, Poisson arrivals at each offered load
+        rates = [float(r) for r in args.serve_rates.split(",")]
+        log("serving: %s rows/s offered, %.1f s each, %d GiB budget" % (args.serve_rates, args.duration, args.budget_gb))
+        if not serve_sids:
+            serve_sids = serving_setups(prover, args.window, args.seed)
+        levels = serve_levels(prover, serve_sids, rates, args.duration, args.budget_gb << 30, args.slots,
+                              args.max_batch_rows, args.seed, stop_past_saturation=False)
+        serving = {"budget_bytes": args.budget_gb << 30, "slots": args.slots,
+                   "max_batch_rows": args.max_batch_rows, "duration_s_per_level": args.duration,
+                   "blobs": "one row each, N=1024/2048/4096 weighted 1:2:5, Poisson arrivals",
+                   "p50_ms": levels[-1]["p50_ms"], "p99_ms": levels[-1]["p99_ms"],
+                   "offered_rows_per_s": levels[-1]["offered_rows_per_s"],
+                   "sustained_rows_per_s": levels[-1]["sustained_rows_per_s"], "levels": levels}
+    for sid in serve_sids.values():
+        prover.unload_setup(sid)
 
     cpu, exact, ok = None, None, None
     if not args.no_cpu:
@@ -856,6 +901,7 @@ def main():
         "latency_ms": {"step_p50": round(ms, 3), "step_p99": round(pct(step_ms, 99), 3)},
         "forms": forms,
         "uneven_batch": uneven,
+        "serving": serving,
         "launch_ms_per_step": {k: round(v, 3) for k, v in sorted(per_step.items(), key=lambda x: -x[1])},
         "setup_table_build_s": round(setup_s, 3),
         "roofline": roofline,

@@ -236,10 +236,12 @@ int fk20_setup_info(fk20_ctx_t *ctx, int setup_id, unsigned *N, unsigned *window
  * fixed batches).  Requests are host polynomial blobs (rows x N of one loaded setup) with a
  * host output buffer; best in pinned memory (fk20_host_alloc) so copies are asynchronous.
  * A worker thread keeps up to `nslots` batches in flight, one per stream; a free slot takes
- * pending requests in arrival order while they fit its share (budget_bytes / nslots) of the
- * device-memory budget for in-flight data -- per row the toeplitz_fft (16*N/8 Fr) and hext
- * (N/8 G1) workspace plus the polynomial and h_fft -- and at most max_batch_rows rows and
- * 16 setups, packs them per setup into one batched pass, and copies results back.
+ * pending requests in arrival order while they fit the free part of the device-memory budget
+ * for in-flight data (one arena of budget_bytes shared by the slots) -- per row the
+ * toeplitz_fft (16*N/8 Fr) and hext (N/8 G1) workspace plus the polynomial and h_fft -- and
+ * at most max_batch_rows rows and 16 setups, packs them per setup into one batched pass, and
+ * copies results back.  Batches are enqueued outside the server's lock, so submit and wait
+ * do not block behind a launch.
  * Requests arriving while batches run are admitted as slots free up; results are the same
  * as proving each request alone.  Outputs use the context's output format as it is when the
  * server is created (host_h_fft then holds rows * N/8 points of that format); batches
@@ -247,7 +249,7 @@ int fk20_setup_info(fk20_ctx_t *ctx, int setup_id, unsigned *N, unsigned *window
 typedef struct fk20_server fk20_server_t;
 int fk20_server_create(fk20_server_t **server, fk20_ctx_t *ctx, size_t budget_bytes, unsigned nslots,
                        unsigned max_batch_rows);
-/* FK20_ENOMEM if the request alone exceeds one slot's share of the budget. */
+/* FK20_ENOMEM if the request alone exceeds the budget (or max_batch_rows). */
 int fk20_server_submit(fk20_server_t *server, int setup_id, const fk20_fr_t *host_polynomial,
                        unsigned rows, fk20_g1p_t *host_h_fft, uint64_t *request_id);
 /* Blocks until the request is done; latency = completion - submission, seconds. */

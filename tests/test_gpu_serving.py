@@ -77,9 +77,41 @@ def test_request_over_budget_is_rejected():
         poly = PinnedArray((1000, 1024, 4))
         out = PinnedArray((1000, 128, 18))
         with pytest.raises(Fk20Error) as ei:
-            srv.submit(sid, poly.ptr, 1000, out.ptr)  # 1000 rows x ~148 KB > 32 MB per slot
+            srv.submit(sid, poly.ptr, 1000, out.ptr)  # 1000 rows x ~148 KB > the 64 MB budget
         assert ei.value.status == -2
         srv.close()
+        poly.free()
+        out.free()
+    finally:
+        p.close()
+
+
+def test_budget_is_shared_by_the_slots():
+    """One arena serves every slot: a request larger than half the budget (the former per-slot
+    share) is admitted whole and proved like the same rows alone; two of them queue behind
+    each other instead of failing."""
+    p = Prover()
+    try:
+        N = 1024
+        tau = S.random_secret(31)
+        sid = p.load_setup(O.xext_fft_from_tau(tau, N), N, 8)
+        rows = 300  # ~44 MB at N = 1024 (about 148 KB per row), within 64 MB, above 32 MB
+        poly = PinnedArray((2, rows, N, 4))
+        poly.array[:] = S.random_polys(2 * rows, N, 32).reshape(2, rows, N, 4)
+        out = PinnedArray((2, rows, N // 8, 18))
+        srv = Server(p, budget_bytes=64 << 20, slots=2, max_batch_rows=4096)
+        ids = [srv.submit(sid, poly.ptr + k * rows * N * 32, rows, out.ptr + k * rows * (N // 8) * 144)
+               for k in range(2)]
+        for i in ids:
+            srv.wait(i)
+        st = srv.stats()
+        srv.close()
+        assert st["batches"] == 2 and st["max_inflight_rows"] == rows
+        alone = p.prove(sid, poly.array[1])
+        assert np.array_equal(out.array[1], alone)
+        exp = O.fk20_closed_form_affine(poly.array[0][:2], tau)
+        bad, first = O.match_affine(out.array[0][:2].reshape(-1, 18), exp.reshape(-1, 12))
+        assert bad == 0
         poly.free()
         out.free()
     finally:
