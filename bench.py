@@ -681,10 +681,12 @@ def main():
     ap.add_argument("--cpu-threads", type=int, default=0,
                     help="CPU baseline threads (0: the CPUs this process may use, at most 16)")
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--streams", type=int, default=2,
-                    help="launch sequences per batch: 2 halves on two streams (default) or 1")
+    ap.add_argument("--streams", type=int, default=0,
+                    help="launch sequences per batch: 2 halves on two streams or 1 (0: the tuned choice)")
     ap.add_argument("--chain", action="store_true",
-                    help="transform stages as one persistent dependency-driven launch (default: one launch per stage)")
+                    help="transform stages as one persistent dependency-driven launch (default: the tuned choice)")
+    ap.add_argument("--schedule", default="tuned",
+                    help="'tuned' (tuned.json for N and rows) or 'wnaf,radix,waves_per_simd', e.g. 5,4,2")
     ap.add_argument("--uneven-rows", type=int, default=512,
                     help="rows per size in the uneven N=1024/2048/4096 batch (0: skip)")
     ap.add_argument("--workload", default="batch", choices=["batch", "serve", "micro", "commit", "fieldmul", "madrate"])
@@ -742,8 +744,16 @@ def main():
     log("generating %d rows, N=%d, setup on device" % (args.rows, args.N))
     b = Batch(args.rows, args.N, args.seed)
     prover = Prover()
-    prover.set_chain(args.chain)
-    prover.set_streams(args.streams)
+    # the transform schedule: the autotuned one for (N, rows) (tuned.json), unless given here
+    from fk20_amd.autotune import apply_schedule, tuned_schedule
+    sch = tuned_schedule(args.N, args.rows) if args.schedule == "tuned" else dict(
+        zip(("wnaf", "radix", "waves_per_simd"), (int(x) for x in args.schedule.split(","))), streams=2, chain=0)
+    if args.streams:
+        sch["streams"] = args.streams
+    if args.chain:
+        sch["chain"] = 1
+    apply_schedule(prover, sch)
+    log("transform schedule: %s" % json.dumps(sch))
     t0 = time.perf_counter()
     sid = prover.load_setup(b.xext, args.N, args.window)
     setup_s = time.perf_counter() - t0
@@ -771,7 +781,8 @@ def main():
     else:
         stages = [None]
     # executed work of this kernel's algorithm per launch (mean over its launches in a step)
-    exe_fp = (sum(WL.executed_launch_fp_muls(args.N, dom, st, args.window) for st in stages)
+    wl_kw = {"wnaf": sch["wnaf"], "r22": sch["radix"] == 4}
+    exe_fp = (sum(WL.executed_launch_fp_muls(args.N, dom, st, args.window, **wl_kw) for st in stages)
               / len(stages) * args.rows)
     work = exe_fp * WL.FP_MUL32
     canon = WL.launch_mul32(args.N, dom, args.rows)
@@ -795,8 +806,8 @@ def main():
         "fp_mul_per_s": round(exe_fp / (avg_ms / 1e3), 1),
         "achieved_canonical": round(canon / (avg_ms / 1e3) / 1e12, 4),
         # the whole step in the benchmarked layout: executed work of every kernel / step time
-        "achieved_step": round(WL.executed_row_fp_muls(args.N, args.window) * args.rows * WL.FP_MUL32
-                               / (ms / 1e3) / 1e12, 4),
+        "achieved_step": round(WL.executed_row_fp_muls(args.N, args.window, chain=bool(sch["chain"]), **wl_kw)
+                               * args.rows * WL.FP_MUL32 / (ms / 1e3) / 1e12, 4),
     }
 
     roofline["valu_counters"] = measured_valu(dom, args.N, args.rows, args.window, avg_ms / 1e3)
@@ -853,10 +864,16 @@ def main():
     uneven, uneven_groups, serve_sids = None, None, {}
     if args.uneven_rows:
         log("timing uneven batch: %d rows each of N=1024/2048/4096 in one pass" % args.uneven_rows)
+        # uneven batches (and the serving loop's mixed batches after them) run the schedule
+        # tuned for the mix
+        msch = tuned_schedule("mixed", 3 * args.uneven_rows) if args.schedule == "tuned" else sch
+        apply_schedule(prover, msch)
+        log("uneven/serving schedule: %s" % json.dumps(msch))
         um, uneven_groups, serve_sids = time_uneven(prover, args.uneven_rows, args.window,
                                                     max(2, args.steps // 2), args.seed, keep=True)
         uneven = {"rows": 3 * args.uneven_rows, "ms_p50": round(statistics.median(um), 3),
-                  "rows_per_s": round(3 * args.uneven_rows / (statistics.median(um) / 1e3), 1)}
+                  "rows_per_s": round(3 * args.uneven_rows / (statistics.median(um) / 1e3), 1),
+                  "schedule": msch}
     serving = None
     if args.serve_rates:
         # the serving loop over the same three resident setups: one-row blobs, N = 1024 / 2048 /
@@ -896,8 +913,8 @@ def main():
         "data": "synthetic: uniform Fr coefficients (PCG64 seed %d), setup from a seeded secret" % args.seed,
         "config": {"workload": "fk20_batch", "rows": args.rows, "N": args.N, "l": 16, "fft_size": args.N // 8,
                    "form": "F batched (fk20_prove_rows)", "msm_window_bits": args.window,
-                   "transform": "persistent chain" if args.chain else "per-stage launches",
-                   "streams": args.streams},
+                   "transform": "persistent chain" if sch["chain"] else "per-stage launches",
+                   "streams": sch["streams"], "schedule": sch},
         "latency_ms": {"step_p50": round(ms, 3), "step_p99": round(pct(step_ms, 99), 3)},
         "forms": forms,
         "uneven_batch": uneven,

@@ -207,7 +207,7 @@ bool append_program(std::vector<uint32_t> &ops, const uint32_t *k, int W, const 
 }
 
 // Builds the programs of w^idx and w^-idx (w of order 512), each checked on the host.
-bool build_programs_host(std::vector<uint32_t> &ops, std::vector<uint32_t> &index) {
+bool build_programs_host(std::vector<uint32_t> &ops, std::vector<uint32_t> &index, int W = TW_WNAF) {
     DevTables T;
     build_host_tables(T);
     ops.clear();
@@ -218,7 +218,7 @@ bool build_programs_host(std::vector<uint32_t> &ops, std::vector<uint32_t> &inde
             index[dir * 512 + idx] = (uint32_t)ops.size();
             fs expect;
             fs_set(expect, T.tw_mont[e]);
-            if (!append_program(ops, T.tw_std[e], TW_WNAF, expect)) return false;
+            if (!append_program(ops, T.tw_std[e], W, expect)) return false;
         }
     }
     return true;
@@ -244,8 +244,9 @@ int status_of(hipError_t e) { return status_from_hip(e); }
 
 namespace {
 // Every program (w^+-i, then 2^-j) on the host, each checked, with its index table.
-bool all_programs_host(std::vector<uint32_t> &ops, std::vector<uint32_t> &index, uint32_t *scale_off) {
-    if (!build_programs_host(ops, index)) return false;
+bool all_programs_host(std::vector<uint32_t> &ops, std::vector<uint32_t> &index, uint32_t *scale_off,
+                       int W = TW_WNAF) {
+    if (!build_programs_host(ops, index, W)) return false;
     DevTables T;
     build_host_tables(T);
     index.resize(TW_INDEX_SIZE);
@@ -253,16 +254,17 @@ bool all_programs_host(std::vector<uint32_t> &ops, std::vector<uint32_t> &index,
         index[TW_SCALE + j] = scale_off[j] = (uint32_t)ops.size();
         fs expect;
         fs_set(expect, T.inv2k_mont[j]);
-        if (!append_program(ops, T.inv2k_std[j], TW_WNAF, expect)) return false;
+        if (!append_program(ops, T.inv2k_std[j], W, expect)) return false;
     }
     return true;
 }
 }  // namespace
 
-hipError_t fk20::build_programs(Programs &P) {
+hipError_t fk20::build_programs(Programs &P, unsigned wnaf) {
     std::vector<uint32_t> ops, index;
-    if (!all_programs_host(ops, index, P.scale_off)) return hipErrorInvalidValue;
+    if (wnaf < 3 || wnaf > 5 || !all_programs_host(ops, index, P.scale_off, (int)wnaf)) return hipErrorInvalidValue;
     P.owned = true;
+    P.wnaf = wnaf;
     hipError_t err = hipMalloc(&P.ops, ops.size() * 4);
     if (err == hipSuccess) err = hipMalloc(&P.index, index.size() * 4);
     if (err == hipSuccess) err = hipMemcpy(P.ops, ops.data(), ops.size() * 4, hipMemcpyHostToDevice);
@@ -280,6 +282,7 @@ hipError_t fk20::load_programs(Programs &P, uint32_t *ops_dev, size_t ops_cap, u
         P.ops = ops_dev;
         P.index = index_dev;
         P.owned = false;
+        P.wnaf = TW_WNAF;
     }
     return err;
 }
@@ -311,6 +314,9 @@ struct fk20_ctx {
     hipStream_t done_stream = nullptr;
     bool done_valid = false;
     int out_format = FK20_OUT_G1P;
+    // schedule of the transform kernels (fk20_ctx_set_schedule); prog is built for sch_wnaf
+    unsigned sch_wnaf = TW_WNAF, sch_occ = DEFAULT_OCCUPANCY;
+    bool sch_r22 = true;
     // fk20_prove_batch splits its rows into `nsub` halves at 64-row boundaries and runs each
     // half's launch sequence on its own stream, forked from and joined back to the caller's
     // stream, so one half's stage launches fill the SIMDs the other half's idle at a boundary
@@ -359,7 +365,7 @@ int fk20_ctx_create(fk20_ctx_t **out) {
     fk20_ctx *c = new fk20_ctx();
     hipError_t e = hipGetDevice(&c->device);
     if (e == hipSuccess) c->T = current_device_tables(&e);
-    if (e == hipSuccess) e = build_programs(c->prog);
+    if (e == hipSuccess) e = build_programs(c->prog, c->sch_wnaf);
     if (e != hipSuccess) {
         fk20_ctx_destroy(c);
         return status_of(e);
@@ -415,6 +421,41 @@ int fk20_ctx_set_streams(fk20_ctx_t *c, unsigned n) {
         c->gkey.clear();
     }
     c->nsub = n;
+    return FK20_OK;
+}
+
+int fk20_ctx_set_schedule(fk20_ctx_t *c, unsigned wnaf, unsigned radix, unsigned waves_per_simd) {
+    if (!c || !stage_kernels(wnaf, radix == 4, waves_per_simd) || (radix != 2 && radix != 4)) return FK20_EINVAL;
+    std::lock_guard<std::mutex> lk(c->mu);
+    if (wnaf != c->sch_wnaf) {
+        // queued passes may still read the old programs
+        Programs np;
+        hipError_t e = build_programs(np, wnaf);
+        if (e == hipSuccess) e = hipDeviceSynchronize();
+        if (e != hipSuccess) {
+            free_programs(np);
+            return status_of(e);
+        }
+        free_programs(c->prog);
+        c->prog = np;
+    }
+    if (c->gexec) {  // captured with the other kernels
+        (void)hipGraphExecDestroy(c->gexec);
+        c->gexec = nullptr;
+        c->gkey.clear();
+    }
+    c->sch_wnaf = wnaf;
+    c->sch_r22 = radix == 4;
+    c->sch_occ = waves_per_simd;
+    return FK20_OK;
+}
+
+int fk20_ctx_get_schedule(fk20_ctx_t *c, unsigned *wnaf, unsigned *radix, unsigned *waves_per_simd) {
+    if (!c) return FK20_EINVAL;
+    std::lock_guard<std::mutex> lk(c->mu);
+    if (wnaf) *wnaf = c->sch_wnaf;
+    if (radix) *radix = c->sch_r22 ? 4 : 2;
+    if (waves_per_simd) *waves_per_simd = c->sch_occ;
     return FK20_OK;
 }
 
@@ -660,7 +701,10 @@ static hipError_t enqueue_launch(fk20_ctx *c, const fk20_batch_item_t *items, un
         it.stage = -1;
     }
     if (chain) c->chain_ctl.push_back(ctl);
-    return launch_batch(L, c->T, c->prog.ops, c->prog.index, s, hook, chain ? ctl : nullptr, c->chain_wait);
+    // the persistent form runs radix-2 stages: the schedule's width and occupancy, radix 2
+    const StageKernels *K = stage_kernels(c->sch_wnaf, chain ? false : c->sch_r22, c->sch_occ);
+    if (!K) return hipErrorInvalidValue;
+    return launch_batch(L, c->T, c->prog.ops, c->prog.index, *K, s, hook, chain ? ctl : nullptr, c->chain_wait);
 }
 
 static hipError_t enqueue_items(fk20_ctx *c, const fk20_batch_item_t *items, unsigned n, hipStream_t s,
@@ -858,8 +902,9 @@ int fk20_twiddle_selfcheck(unsigned *total_ops, unsigned *wnaf_width) {
     return ok ? FK20_OK : FK20_EINVAL;
 }
 
-int fk20_twiddle_program(int which, unsigned idx, uint32_t *ops, unsigned max_ops, unsigned *n_ops) {
-    if (!n_ops || which < 0 || which > 2 || (which < 2 && idx >= 512) || (which == 2 && idx >= 10))
+int fk20_twiddle_program(int which, unsigned idx, unsigned wnaf, uint32_t *ops, unsigned max_ops, unsigned *n_ops) {
+    if (!n_ops || which < 0 || which > 2 || (which < 2 && idx >= 512) || (which == 2 && idx >= 10) || wnaf < 3 ||
+        wnaf > 5)
         return FK20_EINVAL;
     DevTables T;
     build_host_tables(T);
@@ -876,7 +921,7 @@ int fk20_twiddle_program(int which, unsigned idx, uint32_t *ops, unsigned max_op
     fs expect;
     fs_set(expect, km);
     std::vector<uint32_t> prog;
-    if (!append_program(prog, k, TW_WNAF, expect)) return FK20_EINVAL;
+    if (!append_program(prog, k, (int)wnaf, expect)) return FK20_EINVAL;
     *n_ops = (unsigned)prog.size();
     if (ops && max_ops) memcpy(ops, prog.data(), 4 * std::min<size_t>(max_ops, prog.size()));
     return FK20_OK;

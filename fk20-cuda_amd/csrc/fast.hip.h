@@ -21,10 +21,11 @@ struct Programs {
     uint32_t *index = nullptr;
     uint32_t scale_off[10] = {};
     bool owned = false;  // ops and index were allocated by build_programs
+    unsigned wnaf = 0;   // digit width the programs were built for
 };
 // Builds and checks every program on the host (each evaluates to its scalar over Fr with
 // phi = lambda), then copies them to new allocations on the current device.
-hipError_t build_programs(Programs &P);
+hipError_t build_programs(Programs &P, unsigned wnaf = TW_WNAF);
 // The same, copied into caller-owned device memory (ops_cap words of ops, TW_INDEX_SIZE
 // words of index); no allocation.
 hipError_t load_programs(Programs &P, uint32_t *ops_dev, size_t ops_cap, uint32_t *index_dev);
@@ -94,14 +95,37 @@ struct LaunchHook {
     }
 };
 
+// The G1 transform kernels of one schedule variant (stage_kernels.hip.h): wNAF width of the
+// twiddle programs (3..5), radix-2^2 (r22) or radix-2 butterflies, waves per SIMD the
+// register allocation leaves room for (2 or 3).
+// chain is set for the radix-2 variants only (the persistent form runs radix-2 stages).
+struct StageKernels {
+    void (*dit)(Launch, const uint32_t *, const uint32_t *) = nullptr;
+    void (*middle)(Launch, const uint32_t *, const uint32_t *) = nullptr;
+    void (*dif)(Launch, const uint32_t *, const uint32_t *) = nullptr;
+    void (*twiddle)(Launch, const uint32_t *, const uint32_t *, unsigned) = nullptr;
+    void (*chain)(Chain, const uint32_t *, const uint32_t *) = nullptr;
+    void (*plain_load)(uint32_t *, const uint32_t *, unsigned, unsigned, unsigned, const uint32_t *) = nullptr;
+    int wnaf = 0;
+    bool r22 = false;
+    int occupancy = 0;
+};
+// The built variant, or nullptr: wnaf 3..5, r22, waves per SIMD 2..3.
+const StageKernels *stage_kernels(unsigned wnaf, bool r22, unsigned occupancy);
+// The default schedule: the drop-in entry points and new contexts use it.
+static constexpr unsigned DEFAULT_OCCUPANCY = 2;
+inline const StageKernels &default_stage_kernels() { return *stage_kernels(TW_WNAF, true, DEFAULT_OCCUPANCY); }
+
 // The whole batched path for the items of `L`: Toeplitz + NTT, MSM, 2*Lmax - 1 transform
 // stages, output -- one launch per stage covering every item.
 // chain != nullptr: the transform stages run as one persistent k_chain launch whose work
 // counter and error word live at chain[0..1] (cleared by the launch sequence itself); each
 // of its waits gives up after chain_wait ticks and then sets chain[1].
 static constexpr uint64_t CHAIN_WAIT_TICKS = 1ull << 27;  // ~1.3 s at the 100 MHz wall clock
+// K: the transform kernels (K.chain must be set when chain is); ops / index: programs built
+// for K.wnaf.
 hipError_t launch_batch(const Launch &L, const DevTables *T, const uint32_t *ops, const uint32_t *index,
-                        hipStream_t s, LaunchHook hook, uint32_t *chain = nullptr,
+                        const StageKernels &K, hipStream_t s, LaunchHook hook, uint32_t *chain = nullptr,
                         uint64_t chain_wait = CHAIN_WAIT_TICKS);
 
 // The drop-in entry points on the same column-batched transform kernels, for `rows`

@@ -25,48 +25,9 @@
 
 #include "common.hip.h"
 #include "fast.hip.h"
+#include "kutil.hip.h"
 
 namespace fk20 {
-
-// ---------------------------------------------------------------- memory helpers
-
-__device__ __forceinline__ void ld_words(uint32_t *w, const uint32_t *g, int n4) {
-#pragma unroll
-    for (int i = 0; i < n4; i++) {
-        uint4 v = reinterpret_cast<const uint4 *>(g)[i];
-        w[4 * i] = v.x; w[4 * i + 1] = v.y; w[4 * i + 2] = v.z; w[4 * i + 3] = v.w;
-    }
-}
-
-__device__ __forceinline__ void st_words(uint32_t *g, const uint32_t *w, int n4) {
-#pragma unroll
-    for (int i = 0; i < n4; i++)
-        reinterpret_cast<uint4 *>(g)[i] = make_uint4(w[4 * i], w[4 * i + 1], w[4 * i + 2], w[4 * i + 3]);
-}
-
-__device__ __forceinline__ void pt_load(g1p &p, const uint32_t *g) {
-    uint32_t w[36];
-    ld_words(w, g, 9);
-    g1_load_packed(p, w);
-}
-
-__device__ __forceinline__ void pt_store(uint32_t *g, const g1p &p) {
-    uint32_t w[36];
-    g1_store_packed(w, p);
-    st_words(g, w, 9);
-}
-
-__device__ __forceinline__ void fe_load(fe &a, const uint32_t *g) {
-    uint32_t w[12];
-    ld_words(w, g, 3);
-    fe_unpack(a, w);
-}
-
-__device__ __forceinline__ void fe_store(uint32_t *g, const fe &a) {
-    uint32_t w[12];
-    fe_pack(w, a);
-    st_words(g, w, 3);
-}
 
 // ---------------------------------------------------------------- setup: scaled bases
 
@@ -179,22 +140,6 @@ __global__ void __launch_bounds__(64, 3) k_table_build(uint32_t *table, uint32_t
             fe_store(tab + (size_t)e * G1A_WORDS + 12, y);
         }
     }
-}
-
-// ---------------------------------------------------------------- multi-item decoding
-//
-// Every launch below covers a list of items (row groups, each with its own setup, N and
-// buffers).  A block finds its item from the per-item block prefix sums of the launch.
-
-__device__ __forceinline__ unsigned find_item(const Launch &L, unsigned blk, unsigned &local) {
-    unsigned k = 0, start = 0;
-#pragma unroll 1
-    while (k + 1 < L.nitems && blk >= L.blk_end[k]) {
-        start = L.blk_end[k];
-        k++;
-    }
-    local = blk - start;
-    return k;
 }
 
 // ---------------------------------------------------------------- Toeplitz + NTT (transposed)
@@ -402,176 +347,23 @@ __global__ void __launch_bounds__(64) k_commit_reduce(const uint32_t *in, uint32
 
 // ---------------------------------------------------------------- G1 transform stages
 //
-// Per item with L = log2(n2), stage schedule over the 2*Lmax - 1 launches of a batch:
-//   IFFT DIT stage s (0 <= s <= L-2)  at launch Lmax-1-(L-1)+s   (right-aligned)
-//   middle                            at launch Lmax-1            (shared)
-//   FFT DIF stage s (1 <= s <= L-1)   at launch Lmax-1+s          (left-aligned)
-// so every item of a launch runs the same kind; it.stage holds s (or -1: idle).  Within an
-// item, local stage u = 0 .. 2L-2 numbers its own stages: DIT s = u (u < L-1), the middle
-// (u = L-1), DIF s = u - (L-1).  Butterfly b of a stage reads and writes two positions
-// (in place), given by stage_positions().
+// The butterfly, twiddle-layer, persistent-chain and drop-in load kernels are templates over
+// a schedule variant (stage_kernels.hip.h), built once per variant (stage_variant.hip);
+// launches go through the variant's StageKernels table.
 
-// IFFT decimation in time (bit-reversed input): x = D[p1] w^-idx; D[p0], D[p1] = y + x, y - x.
-template <int W>
-__device__ __forceinline__ void dit_butterfly(uint32_t *D, unsigned rows, unsigned row, const Butterfly &f,
-                                              const uint32_t *ops, const uint32_t *index) {
-    g1p x, y, a;
-    pt_load(x, D + ((size_t)f.p1 * rows + row) * 36);
-    if (f.idx) tw_mul<W>(x, ops + index[TW_INV * 512 + f.idx]);
-    pt_load(y, D + ((size_t)f.p0 * rows + row) * 36);
-    g1_add(a, y, x);
-    pt_store(D + ((size_t)f.p0 * rows + row) * 36, a);  // stored first: not live in the next add
-    g1_neg(x, x);
-    g1_add(x, y, x);
-    pt_store(D + ((size_t)f.p1 * rows + row) * 36, x);
+extern const StageKernels stage_kernels_3_0_2, stage_kernels_3_0_3, stage_kernels_3_1_2, stage_kernels_3_1_3,
+    stage_kernels_4_0_2, stage_kernels_4_0_3, stage_kernels_4_1_2, stage_kernels_4_1_3, stage_kernels_5_0_2,
+    stage_kernels_5_0_3, stage_kernels_5_1_2, stage_kernels_5_1_3;
+
+const StageKernels *stage_kernels(unsigned wnaf, bool r22, unsigned occupancy) {
+    static const StageKernels *const table[3][2][2] = {
+        {{&stage_kernels_3_0_2, &stage_kernels_3_0_3}, {&stage_kernels_3_1_2, &stage_kernels_3_1_3}},
+        {{&stage_kernels_4_0_2, &stage_kernels_4_0_3}, {&stage_kernels_4_1_2, &stage_kernels_4_1_3}},
+        {{&stage_kernels_5_0_2, &stage_kernels_5_0_3}, {&stage_kernels_5_1_2, &stage_kernels_5_1_3}}};
+    if (wnaf < 3 || wnaf > 5 || occupancy < 2 || occupancy > 3) return nullptr;
+    return table[wnaf - 3][r22 ? 1 : 0][occupancy - 2];
 }
-
-// Last IFFT stage, upper half dropped (h = first half), then the first FFT DIF stage whose
-// upper input half is zero:  x = E_j + w^-idx O_j;  D[j] = x;  D[j + n2/2] = w^idx x.
-template <int W>
-__device__ __forceinline__ void middle_butterfly(uint32_t *D, unsigned rows, unsigned row, const Butterfly &f,
-                                                 const uint32_t *ops, const uint32_t *index) {
-    g1p o, e;
-    pt_load(o, D + ((size_t)f.p1 * rows + row) * 36);
-    if (f.idx) tw_mul<W>(o, ops + index[TW_INV * 512 + f.idx]);
-    pt_load(e, D + ((size_t)f.p0 * rows + row) * 36);
-    g1_add(e, e, o);
-    pt_store(D + ((size_t)f.p0 * rows + row) * 36, e);
-    if (f.idx) tw_mul<W>(e, ops + index[TW_FWD * 512 + f.idx]);
-    pt_store(D + ((size_t)f.p1 * rows + row) * 36, e);
-}
-
-// FFT decimation in frequency: D[p0], D[p1] = u + v, (u - v) w^idx.
-template <int W>
-__device__ __forceinline__ void dif_butterfly(uint32_t *D, unsigned rows, unsigned row, const Butterfly &f,
-                                              const uint32_t *ops, const uint32_t *index) {
-    g1p u, v, a;
-    pt_load(u, D + ((size_t)f.p0 * rows + row) * 36);
-    pt_load(v, D + ((size_t)f.p1 * rows + row) * 36);
-    g1_add(a, u, v);
-    pt_store(D + ((size_t)f.p0 * rows + row) * 36, a);  // stored first: not live across tw_mul
-    g1_neg(v, v);
-    g1_add(v, u, v);
-    if (f.idx) tw_mul<W>(v, ops + index[TW_FWD * 512 + f.idx]);
-    pt_store(D + ((size_t)f.p1 * rows + row) * 36, v);
-}
-
-// Runs local stage u's butterfly `local` of item `it` for this lane's row.  Lanes past the
-// item's last row repeat the last row's work (same loads, same stores of the same values),
-// so the wave never diverges: the persistent chain below relies on every lane staying
-// converged around its wave-uniform steps.
-template <int W>
-__device__ __forceinline__ void run_butterfly(const Item &it, unsigned u, unsigned local, const uint32_t *ops,
-                                              const uint32_t *index) {
-    const unsigned rows = it.rows, rg = (rows + 63) / 64;
-    const unsigned row = min((local % rg) * 64 + threadIdx.x, rows - 1);
-    const Butterfly f = stage_positions<false>(u, it.logn2, local / rg);
-    if (u + 1 < it.logn2) dit_butterfly<W>(it.D, rows, row, f, ops, index);
-    else if (u + 1 == it.logn2) middle_butterfly<W>(it.D, rows, row, f, ops, index);
-    else dif_butterfly<W>(it.D, rows, row, f, ops, index);
-}
-
-// One launch per stage: it.stage = s of this launch's kind (DIT/DIF), 0 for the middle.
-// Each kernel inlines only its own butterfly (the chain kernel below needs all three).
-// Lanes past the item's last row repeat the last row's work (identical loads and stores
-// within one wave) rather than leaving the wave: a ladder run by a wave with one active
-// lane measured 2.3x slower than the same ladder with all 64 lanes active.
-#define FK20_STAGE_KERNEL(NAME, BODY, USTAGE)                                                       \
-    template <int W>                                                                             \
-    __global__ void __launch_bounds__(64, 2) NAME(Launch L, const uint32_t *ops, const uint32_t *index) { \
-        unsigned local;                                                                          \
-        const Item &it = L.item[find_item(L, blockIdx.x, local)];                                \
-        const unsigned rg = (it.rows + 63) / 64;                                                 \
-        const unsigned row = min((local % rg) * 64 + threadIdx.x, it.rows - 1);                  \
-        const Butterfly f = stage_positions<true>((USTAGE), it.logn2, local / rg);               \
-        BODY<W>(it.D, it.rows, row, f, ops, index);                                              \
-    }
-FK20_STAGE_KERNEL(k_dit_multi, dit_butterfly, (unsigned)it.stage)
-FK20_STAGE_KERNEL(k_middle_multi, middle_butterfly, it.logn2 - 1)
-FK20_STAGE_KERNEL(k_dif_multi, dif_butterfly, it.logn2 - 1 + (unsigned)it.stage)
-#undef FK20_STAGE_KERNEL
-
-// The point-wise twiddle layer of odd DIF stage it.stage (radix-2^2 schedule): after that
-// stage's butterflies in the FFT (dir = TW_FWD), before the butterflies of the transposed DIT
-// stage in the IFFT (dir = TW_INV).  One wave per (nontrivial position, 64-row group).
-template <int W>
-__global__ void __launch_bounds__(64, 2) k_twiddle_multi(Launch L, const uint32_t *ops, const uint32_t *index,
-                                                        unsigned dir) {
-    unsigned local;
-    const Item &it = L.item[find_item(L, blockIdx.x, local)];
-    const unsigned rg = (it.rows + 63) / 64;
-    const unsigned row = min((local % rg) * 64 + threadIdx.x, it.rows - 1);
-    const LayerTw t = layer_twiddle((unsigned)it.stage, it.logn2, local / rg);
-    uint32_t *p = it.D + ((size_t)t.q * it.rows + row) * 36;
-    g1p x;
-    pt_load(x, p);
-    tw_mul<W>(x, ops + index[dir * 512 + t.e]);
-    pt_store(p, x);
-}
-
-// ---------------------------------------------------------------- persistent transform chain
 //
-// All 2*Lmax - 1 stages in one launch of resident waves.  Work items (one butterfly of one
-// 64-row group) are numbered stage by stage; a wave takes the next number from a counter,
-// waits until the two positions it reads carry the flag of the item's previous local stage,
-// runs the butterfly and flags both positions with u + 1.  A wave that finishes a cheap
-// (untwiddled) butterfly moves on to later stages at once instead of idling at a launch
-// boundary.  Items are taken in stage order, so every item an item depends on was taken
-// earlier by a running wave: the waits always end.  Each wait is bounded all the same, by
-// the constant-rate wall clock (C.wait_ticks, by default about 1.3 s at the 100 MHz
-// reference; a legitimate wait is at most one pass, tens of ms); on expiry the wave records
-// an error in ctl[1] and proceeds, and the host turns that word into an error status.
-
-__device__ __forceinline__ unsigned flag_load(const uint32_t *f) {
-    return __hip_atomic_load(f, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-
-template <int W>
-__global__ void __launch_bounds__(64, 2) k_chain(Chain C, const uint32_t *ops, const uint32_t *index) {
-    // No lane-divergent branch anywhere in this loop: the work counter is bumped by lane 0's
-    // addend (every lane executes the atomic, the others add 0) and every lane stores the
-    // flags, so the wave-uniform values below always come from a fully converged wave.  A
-    // wave can take at most `total` items, which bounds the loop whatever happens.
-#pragma unroll 1
-    for (unsigned iter = 0; iter < C.total; iter++) {
-        const unsigned got =
-            __hip_atomic_fetch_add(C.ctl, threadIdx.x == 0 ? 1u : 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        const unsigned w = __builtin_amdgcn_readfirstlane(got);  // lane 0's: the value before its add
-        if (w >= C.total) break;
-        unsigned t = 0;
-#pragma unroll 1
-        while (w >= C.t_end[t]) t++;
-        unsigned local = w - (t ? C.t_end[t - 1] : 0);
-        unsigned k = 0;
-#pragma unroll 1
-        while (local >= C.blk_end[t][k]) k++;
-        if (k) local -= C.blk_end[t][k - 1];
-        const Item &it = C.L.item[k];
-        const unsigned u = (unsigned)C.ustage[t][k];
-        const unsigned rg = (it.rows + 63) / 64, g = local % rg;
-        const Butterfly f = stage_positions<false>(u, it.logn2, local / rg);
-        uint32_t *fl = it.flags + (size_t)g * (1u << it.logn2);
-        if (u) {
-            const uint64_t t0 = wall_clock64();
-            bool expired = false;
-#pragma unroll 1
-            while (flag_load(fl + f.p0) < u || flag_load(fl + f.p1) < u) {
-                if (wall_clock64() - t0 >= C.wait_ticks) {
-                    expired = true;
-                    break;
-                }
-                __builtin_amdgcn_s_sleep(2);
-            }
-            if (expired) __hip_atomic_store(C.ctl + 1, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
-        }
-        run_butterfly<W>(it, u, local, ops, index);
-        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
-        __hip_atomic_store(fl + f.p0, u + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        __hip_atomic_store(fl + f.p1, u + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    }
-}
-
 // Clears the chain's flags of every item and its work counter / error word (ctl[0..1]).
 __global__ void k_chain_reset(Chain C) {
     if (blockIdx.x == 0 && threadIdx.x < 2) C.ctl[threadIdx.x] = 0;
@@ -605,23 +397,6 @@ __global__ void __launch_bounds__(64) k_output_multi(Launch L) {
         g1_store_std(w, p);
         st_words(it.h_fft + slot * G1P_WORDS, w, 9);
     }
-}
-
-// ---------------------------------------------------------------- drop-in entry points
-
-// D[pos][row] = in[row][j] (standard form -> packed Montgomery), pos = bitrev(j) when brev,
-// times the scalar of the program at scale_ops when given.
-template <int W>
-__global__ void __launch_bounds__(64, 2) k_plain_load(uint32_t *D, const uint32_t *in, unsigned rows, unsigned logn,
-                                                     unsigned brev, const uint32_t *scale_ops) {
-    const unsigned rg = (rows + 63) / 64, j = blockIdx.x / rg;
-    const unsigned row = min((blockIdx.x % rg) * 64 + threadIdx.x, rows - 1);  // spare lanes repeat the last row
-    uint32_t w[36];
-    ld_words(w, in + ((size_t)row * (1u << logn) + j) * G1P_WORDS, 9);
-    g1p p;
-    g1_load_std(p, w);
-    if (scale_ops) tw_mul<W>(p, scale_ops);
-    pt_store(D + ((size_t)(brev ? bitrev(j, logn) : j) * rows + row) * 36, p);
 }
 
 // ---------------------------------------------------------------- launchers
@@ -680,7 +455,9 @@ static unsigned set_blocks(Launch &L, const unsigned *per_item) {
 static constexpr unsigned CHAIN_WAVES = 2048;
 
 hipError_t launch_batch(const Launch &base, const DevTables *T, const uint32_t *ops, const uint32_t *index,
-                        hipStream_t s, LaunchHook hook, uint32_t *chain, uint64_t chain_wait) {
+                        const StageKernels &K, hipStream_t s, LaunchHook hook, uint32_t *chain,
+                        uint64_t chain_wait) {
+    if (chain && !K.chain) return hipErrorInvalidValue;
     Launch L = base;
     unsigned per[MAX_ITEMS], lmax = 0;
     for (unsigned k = 0; k < L.nitems; k++) lmax = std::max(lmax, L.item[k].logn2);
@@ -732,24 +509,24 @@ hipError_t launch_batch(const Launch &base, const DevTables *T, const uint32_t *
             C.t_end[t] = acc;
         }
         C.total = acc;
-        if (acc) hipLaunchKernelGGL(k_chain<TW_WNAF>, dim3(std::min(acc, CHAIN_WAVES)), dim3(64), 0, s, C, ops, index);
+        if (acc) hipLaunchKernelGGL(K.chain, dim3(std::min(acc, CHAIN_WAVES)), dim3(64), 0, s, C, ops, index);
         hook(LK_CHAIN);
     } else {
-        // one launch per stage; IFFT DIT stages right-aligned to the middle launch.  A DIT stage
-        // whose transposed DIF stage is odd is preceded by that stage's twiddle layer.
+        // one launch per stage; IFFT DIT stages right-aligned to the middle launch.  In the
+        // radix-2^2 schedule a DIT stage whose transposed DIF stage is odd is preceded by that
+        // stage's twiddle layer.
         for (unsigned t = 0; t + 1 < lmax; t++) {
             for (unsigned k = 0; k < L.nitems; k++) {
                 const Item &it = L.item[k];
                 const int st = (int)t - (int)(lmax - it.logn2);
                 const unsigned sd = it.logn2 - 1 - (unsigned)st;
-                const bool layer = st >= 0 && (sd & 1) && layer_count(sd, it.logn2);
+                const bool layer = K.r22 && st >= 0 && (sd & 1) && layer_count(sd, it.logn2);
                 L.item[k].stage = layer ? (int)sd : -1;
                 per[k] = layer ? layer_count(sd, it.logn2) * ((it.rows + 63) / 64) : 0;
             }
             total = set_blocks(L, per);
             if (total) {
-                hipLaunchKernelGGL(k_twiddle_multi<TW_WNAF>, dim3(total), dim3(64), 0, s, L, ops, index,
-                                   (unsigned)TW_INV);
+                hipLaunchKernelGGL(K.twiddle, dim3(total), dim3(64), 0, s, L, ops, index, (unsigned)TW_INV);
                 hook(LK_TWIDDLE);
             }
             for (unsigned k = 0; k < L.nitems; k++) {
@@ -758,7 +535,7 @@ hipError_t launch_batch(const Launch &base, const DevTables *T, const uint32_t *
                 per[k] = st >= 0 ? (1u << (L.item[k].logn2 - 1)) * ((L.item[k].rows + 63) / 64) : 0;
             }
             total = set_blocks(L, per);
-            if (total) hipLaunchKernelGGL(k_dit_multi<TW_WNAF>, dim3(total), dim3(64), 0, s, L, ops, index);
+            if (total) hipLaunchKernelGGL(K.dit, dim3(total), dim3(64), 0, s, L, ops, index);
             hook(LK_DIT);
         }
         for (unsigned k = 0; k < L.nitems; k++) {
@@ -766,7 +543,7 @@ hipError_t launch_batch(const Launch &base, const DevTables *T, const uint32_t *
             per[k] = (1u << (L.item[k].logn2 - 1)) * ((L.item[k].rows + 63) / 64);
         }
         total = set_blocks(L, per);
-        if (total) hipLaunchKernelGGL(k_middle_multi<TW_WNAF>, dim3(total), dim3(64), 0, s, L, ops, index);
+        if (total) hipLaunchKernelGGL(K.middle, dim3(total), dim3(64), 0, s, L, ops, index);
         hook(LK_MIDDLE);
         // FFT DIF stages 1.., left-aligned after the middle launch
         for (unsigned t = 1; t < lmax; t++) {
@@ -776,19 +553,18 @@ hipError_t launch_batch(const Launch &base, const DevTables *T, const uint32_t *
                 per[k] = st >= 0 ? (1u << (L.item[k].logn2 - 1)) * ((L.item[k].rows + 63) / 64) : 0;
             }
             total = set_blocks(L, per);
-            if (total) hipLaunchKernelGGL(k_dif_multi<TW_WNAF>, dim3(total), dim3(64), 0, s, L, ops, index);
+            if (total) hipLaunchKernelGGL(K.dif, dim3(total), dim3(64), 0, s, L, ops, index);
             hook(LK_DIF);
-            // odd DIF stages: their twiddle layer
+            // odd DIF stages: their twiddle layer (radix-2^2)
             for (unsigned k = 0; k < L.nitems; k++) {
                 const Item &it = L.item[k];
-                const bool layer = t < it.logn2 && (t & 1) && layer_count(t, it.logn2);
+                const bool layer = K.r22 && t < it.logn2 && (t & 1) && layer_count(t, it.logn2);
                 L.item[k].stage = layer ? (int)t : -1;
                 per[k] = layer ? layer_count(t, it.logn2) * ((it.rows + 63) / 64) : 0;
             }
             total = set_blocks(L, per);
             if (total) {
-                hipLaunchKernelGGL(k_twiddle_multi<TW_WNAF>, dim3(total), dim3(64), 0, s, L, ops, index,
-                                   (unsigned)TW_FWD);
+                hipLaunchKernelGGL(K.twiddle, dim3(total), dim3(64), 0, s, L, ops, index, (unsigned)TW_FWD);
                 hook(LK_TWIDDLE);
             }
         }
@@ -824,6 +600,7 @@ hipError_t launch_plain(PlainKind kind, uint32_t *out, const uint32_t *in, const
     const unsigned rg = (rows + 63) / 64, n = 1u << logn;
     unsigned per[MAX_ITEMS] = {};
     const uint32_t *scale = P.ops + P.scale_off[logn];
+    const StageKernels &K = default_stage_kernels();  // radix-2^2, built for P's width
     // radix-2^2 twiddle layers (see stage_positions): DIF stage t's after its butterflies,
     // the transposed DIF stage's before DIT stage t
     auto layer = [&](unsigned sd, unsigned dir) {
@@ -831,11 +608,10 @@ hipError_t launch_plain(PlainKind kind, uint32_t *out, const uint32_t *in, const
         if (!cnt) return;
         it.stage = (int)sd;
         per[0] = cnt * rg;
-        hipLaunchKernelGGL(k_twiddle_multi<TW_WNAF>, dim3(set_blocks(L, per)), dim3(64), 0, s, L, P.ops, P.index,
-                           dir);
+        hipLaunchKernelGGL(K.twiddle, dim3(set_blocks(L, per)), dim3(64), 0, s, L, P.ops, P.index, dir);
     };
     auto stage = [&](void (*kernel)(Launch, const uint32_t *, const uint32_t *), int st) {
-        const bool dit = kernel == k_dit_multi<TW_WNAF>, dif = kernel == k_dif_multi<TW_WNAF>;
+        const bool dit = kernel == K.dit, dif = kernel == K.dif;
         if (dit) layer(logn - 1 - (unsigned)st, (unsigned)TW_INV);
         it.stage = st;
         per[0] = n / 2 * rg;
@@ -845,28 +621,28 @@ hipError_t launch_plain(PlainKind kind, uint32_t *out, const uint32_t *in, const
     };
     switch (kind) {
     case PLAIN_G1_FFT:
-        hipLaunchKernelGGL(k_plain_load<TW_WNAF>, dim3(n * rg), dim3(64), 0, s, D, in, rows, logn, 0u,
+        hipLaunchKernelGGL(K.plain_load, dim3(n * rg), dim3(64), 0, s, D, in, rows, logn, 0u,
                            (const uint32_t *)nullptr);
-        for (unsigned t = 0; t < logn; t++) stage(k_dif_multi<TW_WNAF>, (int)t);
+        for (unsigned t = 0; t < logn; t++) stage(K.dif, (int)t);
         break;
     case PLAIN_G1_IFT:
-        hipLaunchKernelGGL(k_plain_load<TW_WNAF>, dim3(n * rg), dim3(64), 0, s, D, in, rows, logn, 1u, scale);
-        for (unsigned t = 0; t < logn; t++) stage(k_dit_multi<TW_WNAF>, (int)t);
+        hipLaunchKernelGGL(K.plain_load, dim3(n * rg), dim3(64), 0, s, D, in, rows, logn, 1u, scale);
+        for (unsigned t = 0; t < logn; t++) stage(K.dit, (int)t);
         it.out_format = 1;
         break;
     case PLAIN_HEXT_FFT2H_FFT:
     case PLAIN_POLY2H_FFT:
         if (kind == PLAIN_HEXT_FFT2H_FFT) {
-            hipLaunchKernelGGL(k_plain_load<TW_WNAF>, dim3(n * rg), dim3(64), 0, s, D, in, rows, logn, 1u, scale);
+            hipLaunchKernelGGL(K.plain_load, dim3(n * rg), dim3(64), 0, s, D, in, rows, logn, 1u, scale);
         } else {
             per[0] = rows * 16;
             hipLaunchKernelGGL(k_toeplitz_fft_multi, dim3(set_blocks(L, per)), dim3(256), 0, s, L, T);
             hipError_t e = launch_msm_plain_T(D, tcT, xpk, digits, rows, n, s);
             if (e != hipSuccess) return e;
         }
-        for (unsigned t = 0; t + 1 < logn; t++) stage(k_dit_multi<TW_WNAF>, (int)t);
-        stage(k_middle_multi<TW_WNAF>, 0);
-        for (unsigned t = 1; t < logn; t++) stage(k_dif_multi<TW_WNAF>, (int)t);
+        for (unsigned t = 0; t + 1 < logn; t++) stage(K.dit, (int)t);
+        stage(K.middle, 0);
+        for (unsigned t = 1; t < logn; t++) stage(K.dif, (int)t);
         break;
     }
     per[0] = n * rg;

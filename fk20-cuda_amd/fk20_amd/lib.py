@@ -47,7 +47,9 @@ _SIGS = {
     "fk20_poly2h_fft_noalloc": ([_P, _P, _P, _U, _U, _P], _I),
     "fk20_setup2xext_fft": ([_P, _P, _U, _P], _I),
     "fk20_g1p_to_affine": ([_P, _P, _SZ, _P], _I),
-    "fk20_twiddle_program": ([_I, _U, _P, _U, ctypes.POINTER(_U)], _I),
+    "fk20_twiddle_program": ([_I, _U, _U, _P, _U, ctypes.POINTER(_U)], _I),
+    "fk20_ctx_set_schedule": ([_P, _U, _U, _U], _I),
+    "fk20_ctx_get_schedule": ([_P, ctypes.POINTER(_U), ctypes.POINTER(_U), ctypes.POINTER(_U)], _I),
     "fk20_fp_op": ([_I, _P, _P, _P, _SZ, _P], _I),
     "fk20_fr_op": ([_I, _P, _P, _P, _SZ, _P], _I),
     "fk20_g1p_op": ([_I, _P, _P, _P, _P, _SZ, _P], _I),

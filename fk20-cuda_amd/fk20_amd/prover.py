@@ -145,6 +145,16 @@ class Prover:
         halves on two internal streams forked from and joined to the caller's stream."""
         call("fk20_ctx_set_streams", self.ctx, int(n))
 
+    def set_schedule(self, wnaf=5, radix=4, waves_per_simd=2):
+        """Schedule of the transform kernels: wNAF width 3..5 of the twiddle programs, radix 4
+        (radix-2^2 butterflies and twiddle layers) or 2, waves per SIMD 2 or 3."""
+        call("fk20_ctx_set_schedule", self.ctx, int(wnaf), int(radix), int(waves_per_simd))
+
+    def schedule(self):
+        w, r, o = ctypes.c_uint(), ctypes.c_uint(), ctypes.c_uint()
+        call("fk20_ctx_get_schedule", self.ctx, ctypes.byref(w), ctypes.byref(r), ctypes.byref(o))
+        return {"wnaf": w.value, "radix": r.value, "waves_per_simd": o.value}
+
     def set_chain(self, enable):
         """Transform stages as one persistent launch, or one launch per stage (default)."""
         call("fk20_ctx_set_chain", self.ctx, int(bool(enable)))
@@ -190,15 +200,16 @@ def twiddle_selfcheck():
     return status, n.value, w.value
 
 
-def twiddle_program(which, idx):
+def twiddle_program(which, idx, wnaf=5):
     """The ops (uint32 array) of the program the transform kernels run for w^idx (which 0),
-    w^-idx (which 1) or 2^-idx (which 2); host-only, no device needed."""
+    w^-idx (which 1) or 2^-idx (which 2) at wNAF width wnaf; host-only, no device needed."""
     n = ctypes.c_uint()
-    status = lib().fk20_twiddle_program(which, idx, None, 0, ctypes.byref(n))
+    status = lib().fk20_twiddle_program(which, idx, wnaf, None, 0, ctypes.byref(n))
     if status:
         raise ValueError("fk20_twiddle_program(%d, %d): status %d" % (which, idx, status))
     ops = np.zeros(n.value, dtype=np.uint32)
-    status = lib().fk20_twiddle_program(which, idx, ops.ctypes.data_as(ctypes.c_void_p), n.value, ctypes.byref(n))
+    status = lib().fk20_twiddle_program(which, idx, wnaf, ops.ctypes.data_as(ctypes.c_void_p), n.value,
+                                        ctypes.byref(n))
     if status:
         raise ValueError("fk20_twiddle_program(%d, %d): status %d" % (which, idx, status))
     return ops

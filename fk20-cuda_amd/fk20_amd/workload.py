@@ -106,9 +106,17 @@ def launch_mul32(N, kind, rows):
 R_MOD = 0x73EDA753299D7D483339D80809A1D80553BDA402FFFE5BFEFFFFFFFF00000001
 MIXED_ADD = 11
 J_DBL, J_MADD = 7, 11
-WNAF_W = 5  # the library reports its width (fk20_twiddle_selfcheck); a test ties the two
-_NT = 1 << (WNAF_W - 2)
-LADDER_FIXED = 3 + 7 + 6 * (_NT - 1) + 4 * (_NT - 1) + (_NT - 2) + 1 + 4 + 1
+WNAF_W = 5  # the library's default width (fk20_twiddle_selfcheck); a test ties the two
+
+
+def ladder_fixed(wnaf=WNAF_W):
+    """Fp muls of a ladder outside its doublings and additions: conversion in, doubling with
+    update, co-Z additions, rescaling and the table's Z, conversion out, infinity test."""
+    nt = 1 << (wnaf - 2)
+    return 3 + 7 + 6 * (nt - 1) + 4 * (nt - 1) + (nt - 2) + 1 + 4 + 1
+
+
+LADDER_FIXED = ladder_fixed()
 
 TWOP_DBL_MASK, TWOP_PHI, TWOP_PHI2, TWOP_END = 0xFF, 1 << 8, 1 << 13, 1 << 31
 
@@ -124,18 +132,18 @@ def program_counts(ops):
     return dbl, max(len(digits) - 1, 0), phi, len(ops)
 
 
-def twiddle_program(e_idx, inverse=False):
+def twiddle_program(e_idx, inverse=False, wnaf=WNAF_W):
     """(doublings, additions, phi_muls, ops) of the library's program for w^e (w of order 512)."""
-    key = (1 if inverse else 0, e_idx)
+    key = (1 if inverse else 0, e_idx, wnaf)
     if key not in _programs:
         from . import prover
         _programs[key] = program_counts(prover.twiddle_program(*key))
     return _programs[key]
 
 
-def twiddle_fp_muls(e_idx, inverse=False):
-    dbl, adds, phi, _ = twiddle_program(e_idx, inverse)
-    return LADDER_FIXED + dbl * J_DBL + adds * J_MADD + phi
+def twiddle_fp_muls(e_idx, inverse=False, wnaf=WNAF_W):
+    dbl, adds, phi, _ = twiddle_program(e_idx, inverse, wnaf)
+    return ladder_fixed(wnaf) + dbl * J_DBL + adds * J_MADD + phi
 
 
 def _quarter_j(b, lg, lh):
@@ -189,13 +197,30 @@ def r22_layers(N):
     return dit + dif
 
 
-def executed_launch_fp_muls(N, kind, stage=None, window=8):
+def _radix2_idx(u, lg, b):
+    """Twiddle exponent (units of w512) of butterfly b of local stage u in the radix-2
+    schedule (schedule.hip.h stage_positions<false>)."""
+    if u + 1 < lg:
+        return (b & ((1 << u) - 1)) << (8 - u)
+    if u + 1 == lg:
+        return b << (9 - lg)
+    lh = 2 * lg - 2 - u
+    return (b & ((1 << lh) - 1)) << (8 - lh)
+
+
+def executed_launch_fp_muls(N, kind, stage=None, window=8, wnaf=WNAF_W, r22=True):
     """Executed Fp muls of one launch of the batched path, per row.  k_dit_stage /
-    k_dif_stage / k_middle / k_twiddle_layer (stage = (sd, inverse)) follow the radix-2^2
-    schedule of the per-stage launches; k_transform_chain the radix-2 schedule of the
-    persistent chain."""
+    k_dif_stage / k_middle / k_twiddle_layer (stage = (sd, inverse)) follow the schedule of
+    the per-stage launches (radix-2^2 with twiddle layers, or radix 2 with r22 False);
+    k_transform_chain the radix-2 schedule of the persistent chain.  wnaf: the programs'
+    digit width."""
     n2 = N // 8
     lg = int(math.log2(n2))
+    bidx = r22_butterfly_idx if r22 else _radix2_idx
+
+    def tw(e, inverse=False):
+        return twiddle_fp_muls(e, inverse, wnaf)
+
     if kind == "k_msm_comb":
         # ceil(256/w) signed windows per term; zero digits (probability ~2^-w) ignored
         return 16 * n2 * ((256 + window - 1) // window) * MIXED_ADD
@@ -203,41 +228,43 @@ def executed_launch_fp_muls(N, kind, stage=None, window=8):
         u = stage if kind == "k_dit_stage" else lg - 1 + stage
         total = 0
         for b in range(n2 // 2):
-            idx = r22_butterfly_idx(u, lg, b)
-            total += 2 * ADD + (twiddle_fp_muls(idx, inverse=(kind == "k_dit_stage")) if idx else 0)
+            idx = bidx(u, lg, b)
+            total += 2 * ADD + (tw(idx, inverse=(kind == "k_dit_stage")) if idx else 0)
         return total
     if kind == "k_middle":
         total = 0
         for b in range(n2 // 2):
-            idx = r22_butterfly_idx(lg - 1, lg, b)
-            total += ADD + (twiddle_fp_muls(idx, True) + twiddle_fp_muls(idx, False) if idx else 0)
+            idx = bidx(lg - 1, lg, b)
+            total += ADD + (tw(idx, True) + tw(idx, False) if idx else 0)
         return total
     if kind == "k_twiddle_layer":
         sd, inverse = stage
-        return sum(twiddle_fp_muls(e, inverse) for _, e in r22_layer(sd, lg))
+        return sum(tw(e, inverse) for _, e in r22_layer(sd, lg))
     if kind == "k_transform_chain":
         total = 0
-        for s in range(lg - 1):  # radix-2 DIT stages
+        for u in range(lg - 1):  # radix-2 DIT stages
             for b in range(n2 // 2):
-                idx = (b & ((1 << s) - 1)) << (8 - s)
-                total += 2 * ADD + (twiddle_fp_muls(idx, True) if idx else 0)
+                idx = _radix2_idx(u, lg, b)
+                total += 2 * ADD + (tw(idx, True) if idx else 0)
         for j in range(n2 // 2):  # middle
-            idx = j << (9 - lg)
-            total += ADD + (twiddle_fp_muls(idx, True) + twiddle_fp_muls(idx, False) if idx else 0)
-        for s in range(1, lg):  # radix-2 DIF stages
-            lh = lg - s - 1
+            idx = _radix2_idx(lg - 1, lg, j)
+            total += ADD + (tw(idx, True) + tw(idx, False) if idx else 0)
+        for u in range(lg, 2 * lg - 1):  # radix-2 DIF stages
             for b in range(n2 // 2):
-                idx = (b & ((1 << lh) - 1)) << (8 - lh)
-                total += 2 * ADD + (twiddle_fp_muls(idx) if idx else 0)
+                idx = _radix2_idx(u, lg, b)
+                total += 2 * ADD + (tw(idx) if idx else 0)
         return total
     raise KeyError(kind)
 
 
-def executed_row_fp_muls(N, window=8):
+def executed_row_fp_muls(N, window=8, wnaf=WNAF_W, r22=True, chain=False):
     lg = int(math.log2(N // 8))
     t = executed_launch_fp_muls(N, "k_msm_comb", window=window)
-    t += sum(executed_launch_fp_muls(N, "k_dit_stage", s) for s in range(lg - 1))
-    t += executed_launch_fp_muls(N, "k_middle")
-    t += sum(executed_launch_fp_muls(N, "k_dif_stage", s) for s in range(1, lg))
-    t += sum(executed_launch_fp_muls(N, "k_twiddle_layer", st) for st in r22_layers(N))
+    if chain:
+        return t + executed_launch_fp_muls(N, "k_transform_chain", wnaf=wnaf)
+    t += sum(executed_launch_fp_muls(N, "k_dit_stage", s, wnaf=wnaf, r22=r22) for s in range(lg - 1))
+    t += executed_launch_fp_muls(N, "k_middle", wnaf=wnaf, r22=r22)
+    t += sum(executed_launch_fp_muls(N, "k_dif_stage", s, wnaf=wnaf, r22=r22) for s in range(1, lg))
+    if r22:
+        t += sum(executed_launch_fp_muls(N, "k_twiddle_layer", st, wnaf=wnaf) for st in r22_layers(N))
     return t

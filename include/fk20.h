@@ -166,6 +166,19 @@ int fk20_ctx_set_chain_wait(fk20_ctx_t *ctx, uint64_t ticks);
  * 1: a single sequence on the caller's stream.  Results are identical.  fk20_prove_batch_ws
  * (the serving loop's, which overlaps batches itself) always uses the caller's stream. */
 int fk20_ctx_set_streams(fk20_ctx_t *ctx, unsigned n);
+/* Schedule of the transform kernels, per context (the choices an autotuner sweeps; results
+ * are identical for every schedule):
+ *   wnaf            3, 4 or 5: digit width of the twiddle programs, i.e. a table of 2, 4 or 8
+ *                   odd multiples per ladder (default 5)
+ *   radix           4: radix-2^2 butterflies (only w^+-128 twiddles) with point-wise twiddle
+ *                   layers (default); 2: radix-2 butterflies carrying every twiddle
+ *   waves_per_simd  2 (default): kernels compiled to leave room for two waves per SIMD
+ *                   (up to 256 registers each); 3: three (168 registers, the rest spilled)
+ * The reference's counterparts are compile-time switches (FFT_Sharedmem, fk20benchmark.cu:302;
+ * SHAREDMEMROOTS, fk20_hext_fft2h_fft.cu:4; CUDAGRAPH, fk20benchmark.cu:5).  The persistent
+ * form (fk20_ctx_set_chain) always runs radix-2 stages with the schedule's width and waves. */
+int fk20_ctx_set_schedule(fk20_ctx_t *ctx, unsigned wnaf, unsigned radix, unsigned waves_per_simd);
+int fk20_ctx_get_schedule(fk20_ctx_t *ctx, unsigned *wnaf, unsigned *radix, unsigned *waves_per_simd);
 /* Output format of fk20_prove_rows / fk20_prove_batch (h_fft buffers), per context:
  *   FK20_OUT_G1P           g1p_t[rows][n2], natural order: the reference ABI (default)
  *   FK20_OUT_G1P_DA        g1p_t, data-availability order: FK20Py list_to_reverse_bit_order
@@ -274,11 +287,11 @@ int fk20_ctx_launch_times(fk20_ctx_t *ctx, float *ms, int *kinds, int max, int *
  * the programs (and the transform kernels' tables of odd multiples) use.  Needs no device. */
 int fk20_twiddle_selfcheck(unsigned *total_ops, unsigned *wnaf_width);
 /* Host-only copy of one program the transform kernels run: which = 0 for w^idx, 1 for
- * w^-idx (idx < 512), 2 for 2^-idx (idx < 10).  Writes at most max_ops 32-bit ops (the
- * encoding of ladder.hip.h) and the program's length to n_ops.  Needs no device.  No
- * reference counterpart (the reference multiplies by twiddles with double-and-add,
- * g1p_mul.cu:18). */
-int fk20_twiddle_program(int which, unsigned idx, uint32_t *ops, unsigned max_ops, unsigned *n_ops);
+ * w^-idx (idx < 512), 2 for 2^-idx (idx < 10), for wNAF width wnaf (3..5).  Writes at most
+ * max_ops 32-bit ops (the encoding of ladder.hip.h) and the program's length to n_ops.  Needs
+ * no device.  No reference counterpart (the reference multiplies by twiddles with
+ * double-and-add, g1p_mul.cu:18). */
+int fk20_twiddle_program(int which, unsigned idx, unsigned wnaf, uint32_t *ops, unsigned max_ops, unsigned *n_ops);
 
 /* Measurement helper: blocks x 256 lanes each issue 8 x iters dependent-free 32 x 32 -> 64-bit
  * multiply-adds (v_mad_u64_u32, the instruction every Fp product is built from); out receives

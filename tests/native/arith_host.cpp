@@ -159,6 +159,16 @@ void host_tw_mul(uint64_t *r, const uint64_t *p, const uint32_t *ops) {
     stp(r, a);
 }
 
+// The same ladder at table width w (3..5), for programs built at that width.
+void host_tw_mul_w(uint64_t *r, const uint64_t *p, const uint32_t *ops, int w) {
+    g1p a;
+    ldp(a, p);
+    if (w == 3) tw_mul<3>(a, ops);
+    else if (w == 4) tw_mul<4>(a, ops);
+    else tw_mul<5>(a, ops);
+    stp(r, a);
+}
+
 // Jacobian doubling / mixed addition, g1p_t slots holding Jacobian (X, Y, Z) and the affine
 // operand (x, y) in the first two coordinates; `reps` repeats the operation on its output.
 void host_g1j_dbl(uint64_t *r, const uint64_t *p, int reps) {

@@ -234,3 +234,24 @@ def test_ladder_infinity_and_structured_inputs(L):
             r = np.zeros(18, dtype=np.uint64)
             L.host_tw_mul(p64(r), p64(p), ops.ctypes.data_as(ctypes.c_void_p))
             assert O.g1p_eq(r, O.g1p_op("mul", G, O.fr(k * m % R)))
+
+
+@pytest.mark.parametrize("w", [3, 4])
+def test_ladder_other_widths(L, w):
+    """The schedule variants' narrower tables: programs built at wNAF width 3 and 4 (2 and 4
+    odd multiples) run by the ladder of that width, a sample of twiddles and every 2^-j."""
+    from fk20_amd import prover
+    rng = np.random.default_rng(44 + w)
+    progs = [(which, idx) for which in (0, 1) for idx in (1, 2, 3, 64, 128, 129, 255, 256, 384, 511)]
+    progs += [(2, j) for j in range(10)]
+    s = S.random_fr(len(progs), 45 + w)
+    zs = np.array([O.fp(int.from_bytes(rng.bytes(48), "little") % (P - 1) + 1) for _ in progs])
+    pts = O.gen_mul_projective(s, zs)
+    exp = O.gen_mul_affine(np.array([O.fr(_program_scalar(a, b) * O.limbs_to_int(si) % R)
+                                     for (a, b), si in zip(progs, s)]))
+    out = np.zeros((len(progs), 18), dtype=np.uint64)
+    for n, (a, b) in enumerate(progs):
+        ops = prover.twiddle_program(a, b, w)
+        L.host_tw_mul_w(p64(out[n]), p64(pts[n]), ops.ctypes.data_as(ctypes.c_void_p), w)
+    bad, first = O.match_affine(out, exp)
+    assert bad == 0, "program %s differs" % (progs[first],)

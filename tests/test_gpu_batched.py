@@ -171,6 +171,44 @@ def test_chain_equals_per_stage_launches(prover):
         prover.unload_setup(sid)
 
 
+def test_every_schedule_variant_gives_the_same_points(prover):
+    """Each transform schedule the autotuner may pick -- wNAF width 3 / 4 / 5, radix-2^2 or
+    radix-2 butterflies, two or three waves per SIMD, and the persistent chain at each width --
+    proves an uneven batch (partial 64-row groups, three sizes) to the same points as the
+    default schedule (canonical affine), and the default matches the closed form."""
+    groups, taus = [], []
+    for N, rows, seed in ((1024, 70, 411), (4096, 3, 412), (2048, 65, 413)):
+        tau = S.random_secret(seed)
+        sid = prover.load_setup(O.xext_fft_from_tau(tau, N), N, 8)
+        groups.append((sid, S.random_polys(rows, N, seed + 10)))
+        taus.append(tau)
+    base = [aff(x) for x in prover.prove_batch(groups)]
+    for (sid, poly), tau, b in zip(groups, taus, base):
+        r = poly.shape[0] - 1
+        assert np.array_equal(b[r * (b.shape[0] // poly.shape[0]):], aff(shadow_points(poly[r], tau)))
+    try:
+        for wnaf in (3, 4, 5):
+            for radix in (2, 4):
+                for waves in (2, 3):
+                    prover.set_schedule(wnaf, radix, waves)
+                    assert prover.schedule() == {"wnaf": wnaf, "radix": radix, "waves_per_simd": waves}
+                    got = prover.prove_batch(groups)
+                    for a, b in zip(got, base):
+                        assert np.array_equal(aff(a), b), (wnaf, radix, waves)
+            prover.set_schedule(wnaf, 4, 2)
+            prover.set_chain(True)
+            try:
+                got = prover.prove_batch(groups)
+            finally:
+                prover.set_chain(False)
+            for a, b in zip(got, base):
+                assert np.array_equal(aff(a), b), (wnaf, "chain")
+    finally:
+        prover.set_schedule(5, 4, 2)
+    for sid, _ in groups:
+        prover.unload_setup(sid)
+
+
 def test_two_stream_split_equals_single_stream(prover):
     """The default two-halves launch layout gives bit-identical outputs to one launch sequence,
     for items of one, two and three 64-row groups in one batch, directly and from a graph."""

@@ -92,8 +92,19 @@ def test_executed_work_model_matches_library_programs():
     # w^128 = z lambda + z: 64-bit halves, about 62 doublings
     dbl, adds, _, _ = workload.twiddle_program(128)
     assert 55 <= dbl <= 66 and adds <= 30
-    for which, idx in ((0, 512), (1, 512), (2, 10), (3, 0), (-1, 0)):
-        assert L.lib().fk20_twiddle_program(which, idx, None, 0, ctypes.byref(ctypes.c_uint())) == -1
+    n = ctypes.c_uint()
+    for which, idx, w in ((0, 512, 5), (1, 512, 5), (2, 10, 5), (3, 0, 5), (-1, 0, 5), (0, 1, 2), (0, 1, 6)):
+        assert L.lib().fk20_twiddle_program(which, idx, w, None, 0, ctypes.byref(n)) == -1
+    # narrower tables mean more additions: wNAF-3 programs are longer than wNAF-5 ones
+    lens = [sum(len(prover.twiddle_program(0, e, w)) for e in range(1, 512, 37)) for w in (3, 4, 5)]
+    assert lens[0] > lens[1] > lens[2]
+
+
+def test_schedule_arguments_checked_before_device():
+    lib = L.lib()
+    for w, r, o in ((2, 4, 2), (6, 4, 2), (5, 3, 2), (5, 4, 1), (5, 4, 4)):
+        assert lib.fk20_ctx_set_schedule(None, w, r, o) == -1
+    assert lib.fk20_ctx_get_schedule(None, None, None, None) == -1
 
 
 def test_noalloc_range_check_before_device():
