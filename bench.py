@@ -401,6 +401,57 @@ def bench_madrate(args):
                       "frac_of_assumed_peak": round(rate / WL.PEAK_MUL32_PER_S, 4)}))
 
 
+def bench_limbs(args):
+    """The Fp multiplication in the library's 14 x 29-bit product-scanning layout against a
+    12 x 32-bit CIOS one (fk20_fp_mul_probe): 8 blocks of 256 lanes per CU, two independent
+    chains of multiplications per lane; both results checked against Python integers on a
+    sample of lanes."""
+    P = 0x1A0111EA397FE69A4B1BA7B6434BACD764774B84F38512BF6730D2A0F6B0F6241EABFFFEB153FFFFB9FEFFFFFFFFAAAB
+    rng = np.random.default_rng(args.seed)
+    xs = [int.from_bytes(rng.bytes(48), "little") % P for _ in range(1024)]
+    yv = int.from_bytes(rng.bytes(48), "little") % P
+
+    def words(v):
+        return np.array([(v >> (32 * i)) & 0xFFFFFFFF for i in range(12)], dtype=np.uint32)
+
+    xb = L.DeviceBuffer.from_array(np.stack([words(x) for x in xs]))
+    yb = L.DeviceBuffer.from_array(words(yv))
+    blocks, iters = 256 * 8, 2000
+    out = L.DeviceBuffer(blocks * 256 * 48)
+    stream = L.Stream()
+    res = {}
+    for limbs, rbits in ((29, 406), (32, 384)):
+        L.call("fk20_fp_mul_probe", limbs, out.ptr, xb.ptr, yb.ptr, blocks, 10, stream.handle)
+        stream.synchronize()
+        ms = statistics.median(time_calls(lambda: L.call("fk20_fp_mul_probe", limbs, out.ptr, xb.ptr, yb.ptr, blocks,
+                                                         iters, stream.handle), args.steps, stream))
+        got = out.download((blocks * 256, 12), np.uint32)
+        rinv = pow(2, -rbits, P)
+        ok = True
+        for lane in range(0, blocks * 256, blocks * 256 // 64):  # 64 sample lanes
+            a = xs[lane & 1023]
+            # chains: a y^iters and (a y) y^iters, each product times R^-1, then their product
+            c1 = a * pow(yv * rinv, iters, P) % P
+            c2 = a * yv * rinv % P * pow(yv * rinv, iters, P) % P
+            v = sum(int(w) << (32 * i) for i, w in enumerate(got[lane]))
+            ok = ok and v % P == c1 * c2 * rinv % P and v < 2 * P
+        muls = blocks * 256 * (2 * iters + 2)
+        res[limbs] = {"ms_p50": round(ms, 3), "fp_mul_per_s": round(muls / (ms / 1e3), 1), "exact": ok}
+        log("limbs %d: %.3f ms, %.1f G Fp mul/s, exact %s" % (limbs, ms, muls / (ms / 1e3) / 1e9, ok))
+    for b in (xb, yb, out):
+        b.free()
+    stream.destroy()
+    print(json.dumps({"metric": "Fp Montgomery multiplication rate by limb layout (fk20_fp_mul_probe)",
+                      "value": res[29]["fp_mul_per_s"], "unit": "Fp mul/s", "n_gpus": 1, "steps": args.steps,
+                      "higher_is_better": True, "dtype": "u32", "data": "synthetic",
+                      "config": {"workload": "fp_limbs", "blocks": blocks, "lanes_per_block": 256,
+                                 "chains_per_lane": 2, "iters": iters},
+                      "layouts": {"14x29_product_scanning": res[29], "12x32_cios": res[32]},
+                      "ratio_29_over_32": round(res[29]["fp_mul_per_s"] / res[32]["fp_mul_per_s"], 3)}))
+    if not (res[29]["exact"] and res[32]["exact"]):
+        sys.exit("limb probe results differ from Python integers")
+
+
 def bench_fieldmul(args):
     """Throughput of the Fp Montgomery multiplication itself (14 x 29-bit limbs), the unit
     every G1 kernel is built from: Fermat inversions a^(p-2) through fk20_fp_op (one lane per
@@ -689,7 +740,8 @@ def main():
                     help="'tuned' (tuned.json for N and rows) or 'wnaf,radix,waves_per_simd', e.g. 5,4,2")
     ap.add_argument("--uneven-rows", type=int, default=512,
                     help="rows per size in the uneven N=1024/2048/4096 batch (0: skip)")
-    ap.add_argument("--workload", default="batch", choices=["batch", "serve", "micro", "commit", "fieldmul", "madrate"])
+    ap.add_argument("--workload", default="batch",
+                    choices=["batch", "serve", "micro", "commit", "fieldmul", "madrate", "limbs"])
     ap.add_argument("--fieldmul-count", type=int, default=1 << 21, help="fieldmul: elements inverted per call")
     ap.add_argument("--calibrate-copy-mb", type=int, default=0,
                     help="before timing, one device-to-device copy of this many MiB (the known-size "
@@ -729,6 +781,9 @@ def main():
         return
     if args.workload == "madrate":
         bench_madrate(args)
+        return
+    if args.workload == "limbs":
+        bench_limbs(args)
         return
 
     if args.calibrate_copy_mb:

@@ -384,6 +384,12 @@ int fk20_mad_probe(uint64_t *out, unsigned blocks, unsigned iters, fk20_stream_t
     return hip_status(launch_mad_probe(out, blocks, iters, S(stream)));
 }
 
+int fk20_fp_mul_probe(int limbs, uint32_t *out, const uint32_t *xs, const uint32_t *y, unsigned blocks,
+                      unsigned iters, fk20_stream_t stream) {
+    if ((limbs != 29 && limbs != 32) || !out || !xs || !y || !blocks) return FK20_EINVAL;
+    return hip_status(launch_fp_mul_probe(limbs, out, xs, y, blocks, iters, S(stream)));
+}
+
 int fk20_g1p_to_affine(fk20_g1a_t *out, const fk20_g1p_t *in, size_t count, fk20_stream_t stream) {
     if (count && (!out || !in)) return FK20_EINVAL;
     return hip_status(launch_to_affine(W(out), W(in), count, S(stream)));

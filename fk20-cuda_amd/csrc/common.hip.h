@@ -60,6 +60,10 @@ hipError_t launch_msm_plain_T(uint32_t *D, const uint32_t *tcT, const uint32_t *
                               unsigned rows, unsigned n2, hipStream_t s);
 // blocks x 256 lanes, each 8 x iters v_mad_u64_u32; out receives blocks * 256 words.
 hipError_t launch_mad_probe(uint64_t *out, unsigned blocks, unsigned iters, hipStream_t s);
+// blocks x 256 lanes, two chains of iters Fp multiplications each in the 14 x 29-bit (limbs 29)
+// or 12 x 32-bit CIOS (limbs 32) layout; out receives blocks * 256 * 12 words.
+hipError_t launch_fp_mul_probe(int limbs, uint32_t *out, const uint32_t *xs, const uint32_t *y, unsigned blocks,
+                               unsigned iters, hipStream_t s);
 hipError_t launch_setup_gather(uint32_t *xext, const uint32_t *srs, unsigned N, hipStream_t s);
 hipError_t launch_to_affine(uint32_t *out, const uint32_t *in, size_t count, hipStream_t s);
 hipError_t launch_compress(uint32_t *out, const uint32_t *in, size_t count, hipStream_t s);

@@ -264,6 +264,87 @@ __global__ void __launch_bounds__(256) k_mad_probe(uint64_t *out, unsigned iters
     out[(size_t)blockIdx.x * 256 + threadIdx.x] = h;
 }
 
+// ---------------------------------------------------------------- limb-layout probe
+//
+// The Fp multiplication in the two limb layouts considered for the library: LIMBS29 = the
+// library's own 14 x 29-bit product scanning (fp.hip.h fe_mul, Montgomery radix 2^406, one
+// v_mad_u64_u32 per partial product into a carry-free 64-bit column), LIMBS32 = 12 x 32-bit
+// CIOS (Koc, Acar, Kaliski 1996; radix 2^384), where each partial product needs a 64-bit
+// multiply-add plus a carry addition.  Both lazy: inputs and outputs below 2p.
+
+// z = a b 2^-384 mod p, 12 x 32-bit CIOS; a, b < 2p (4p < 2^384, so no final subtraction).
+__device__ __forceinline__ void fp32_mul_cios(uint32_t *z, const uint32_t *a, const uint32_t *b) {
+    uint32_t t[14];
+#pragma unroll
+    for (int j = 0; j < 14; j++) t[j] = 0;
+#pragma unroll
+    for (int i = 0; i < 12; i++) {
+        uint64_t c = 0;
+#pragma unroll
+        for (int j = 0; j < 12; j++) {
+            const uint64_t s = (uint64_t)a[j] * b[i] + t[j] + c;
+            t[j] = (uint32_t)s;
+            c = s >> 32;
+        }
+        uint64_t s = (uint64_t)t[12] + c;
+        t[12] = (uint32_t)s;
+        t[13] = (uint32_t)(s >> 32);
+        const uint32_t m = t[0] * FP32_PINV;
+        s = (uint64_t)m * FP32_P[0] + t[0];
+        c = s >> 32;
+#pragma unroll
+        for (int j = 1; j < 12; j++) {
+            s = (uint64_t)m * FP32_P[j] + t[j] + c;
+            t[j - 1] = (uint32_t)s;
+            c = s >> 32;
+        }
+        s = (uint64_t)t[12] + c;
+        t[11] = (uint32_t)s;
+        t[12] = t[13] + (uint32_t)(s >> 32);
+    }
+#pragma unroll
+    for (int j = 0; j < 12; j++) z[j] = t[j];
+}
+
+// Lane l takes x = xs[l mod 1024] and the shared y, runs two independent chains of `iters`
+// multiplications by y, and writes their product (12 words, packed).
+template <int LIMBS>
+__global__ void __launch_bounds__(256) k_fp_mul_probe(uint32_t *out, const uint32_t *xs, const uint32_t *y,
+                                                      unsigned iters) {
+    const size_t lane = (size_t)blockIdx.x * 256 + threadIdx.x;
+    const uint32_t *x = xs + (lane & 1023) * 12;
+    if (LIMBS == 29) {
+        fe a, b, c;
+        fe_unpack(a, x);
+        fe_unpack(b, y);
+        c = a;
+        fe_mul(c, c, b);  // a second chain, offset by one step
+#pragma unroll 1
+        for (unsigned it = 0; it < iters; it++) {
+            fe_mul(a, a, b);
+            fe_mul(c, c, b);
+        }
+        fe_mul(a, a, c);
+        fe_pack(out + lane * 12, a);
+    } else {
+        uint32_t a[12], b[12], c[12];
+#pragma unroll
+        for (int j = 0; j < 12; j++) {
+            a[j] = x[j];
+            b[j] = y[j];
+        }
+        fp32_mul_cios(c, a, b);
+#pragma unroll 1
+        for (unsigned it = 0; it < iters; it++) {
+            fp32_mul_cios(a, a, b);
+            fp32_mul_cios(c, c, b);
+        }
+        fp32_mul_cios(a, a, c);
+#pragma unroll
+        for (int j = 0; j < 12; j++) out[lane * 12 + j] = a[j];
+    }
+}
+
 // xext[i][m] = srs[N-17-i-16m] for m <= k-2, infinity otherwise (FK20Py fk20_multi.py:82-85:
 // setup[n-l-1-i::-l] + [Z1], padded to 2k with Z1).
 __global__ void k_setup_gather(uint32_t *xext, const uint32_t *srs, unsigned N) {
@@ -478,6 +559,13 @@ hipError_t launch_msm_plain_T(uint32_t *D, const uint32_t *tcT, const uint32_t *
 
 hipError_t launch_mad_probe(uint64_t *out, unsigned blocks, unsigned iters, hipStream_t s) {
     hipLaunchKernelGGL(k_mad_probe, dim3(blocks), dim3(256), 0, s, out, iters);
+    return hipGetLastError();
+}
+
+hipError_t launch_fp_mul_probe(int limbs, uint32_t *out, const uint32_t *xs, const uint32_t *y, unsigned blocks,
+                               unsigned iters, hipStream_t s) {
+    if (limbs == 29) hipLaunchKernelGGL(k_fp_mul_probe<29>, dim3(blocks), dim3(256), 0, s, out, xs, y, iters);
+    else hipLaunchKernelGGL(k_fp_mul_probe<32>, dim3(blocks), dim3(256), 0, s, out, xs, y, iters);
     return hipGetLastError();
 }
 

@@ -48,6 +48,7 @@ _SIGS = {
     "fk20_setup2xext_fft": ([_P, _P, _U, _P], _I),
     "fk20_g1p_to_affine": ([_P, _P, _SZ, _P], _I),
     "fk20_twiddle_program": ([_I, _U, _U, _P, _U, ctypes.POINTER(_U)], _I),
+    "fk20_fp_mul_probe": ([_I, _P, _P, _P, _U, _U, _P], _I),
     "fk20_ctx_set_schedule": ([_P, _U, _U, _U], _I),
     "fk20_ctx_get_schedule": ([_P, ctypes.POINTER(_U), ctypes.POINTER(_U), ctypes.POINTER(_U)], _I),
     "fk20_fp_op": ([_I, _P, _P, _P, _SZ, _P], _I),

@@ -298,6 +298,14 @@ int fk20_twiddle_program(int which, unsigned idx, unsigned wnaf, uint32_t *ops, 
  * blocks * 256 words that depend on every result.  Time it to obtain the device's
  * multiply-add issue rate.  No reference counterpart. */
 int fk20_mad_probe(uint64_t *out, unsigned blocks, unsigned iters, fk20_stream_t stream);
+/* Measurement helper behind the choice of limb layout: blocks x 256 lanes, lane l starting
+ * from x = xs[l mod 1024] (12 words, < 2p), run two independent chains of iters Montgomery
+ * multiplications by y (< 2p) and write their product (12 words per lane, < 2p) to out.
+ * limbs = 29: the library's 14 x 29-bit product scanning (radix 2^406); limbs = 32: 12 x 32-bit
+ * CIOS (radix 2^384).  No reference counterpart (the reference multiplies with inline PTX,
+ * fp_mul.cuh:13). */
+int fk20_fp_mul_probe(int limbs, uint32_t *out, const uint32_t *xs, const uint32_t *y, unsigned blocks,
+                      unsigned iters, fk20_stream_t stream);
 
 /* ------------------------------------------------------------------ helpers */
 

@@ -242,3 +242,36 @@ def test_zero_rows_gpu(case1024):
     N, _, _, xext = case1024
     got = ops.poly2h_fft(np.zeros((1, N, 4), dtype=np.uint64), xext)
     assert all(O.g1p_is_inf(p) for p in got[0])
+
+
+@pytest.mark.parametrize("limbs,rbits", [(29, 406), (32, 384)])
+def test_fp_mul_probe_layouts_exact(limbs, rbits):
+    """Both limb layouts of fk20_fp_mul_probe (the measurement behind the library's choice of
+    14 x 29-bit limbs) compute exact Montgomery products: every lane of one block against
+    Python integers."""
+    from fk20_amd.lib import DeviceBuffer, call, synchronize
+    P = O.P_MOD
+    rng = np.random.default_rng(limbs)
+    xs = [int.from_bytes(rng.bytes(48), "little") % P for _ in range(1024)]
+    xs[1] = P - 1
+    y = int.from_bytes(rng.bytes(48), "little") % P
+
+    def words(v):
+        return np.array([(v >> (32 * i)) & 0xFFFFFFFF for i in range(12)], dtype=np.uint32)
+
+    xb = DeviceBuffer.from_array(np.stack([words(x) for x in xs]))
+    yb = DeviceBuffer.from_array(words(y))
+    out = DeviceBuffer(256 * 48)
+    iters = 5
+    call("fk20_fp_mul_probe", limbs, out.ptr, xb.ptr, yb.ptr, 1, iters, None)
+    synchronize()
+    got = out.download((256, 12), np.uint32)
+    for b in (xb, yb, out):
+        b.free()
+    rinv = pow(2, -rbits, P)
+    for lane in range(256):
+        a = xs[lane]
+        c1 = a * pow(y * rinv, iters, P) % P
+        c2 = a * y * rinv % P * pow(y * rinv, iters, P) % P
+        v = sum(int(w) << (32 * i) for i, w in enumerate(got[lane]))
+        assert v % P == c1 * c2 * rinv % P and v < 2 * P, lane

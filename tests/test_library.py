@@ -53,6 +53,9 @@ def test_binding_covers_header():
     lambda l: l.fk20_g1p_op(3, 1, 1, None, None, 1, None),
     lambda l: l.fk20_mad_probe(None, 1, 1, None),
     lambda l: l.fk20_mad_probe(1, 0, 1, None),
+    lambda l: l.fk20_fp_mul_probe(30, 16, 16, 16, 1, 1, None),
+    lambda l: l.fk20_fp_mul_probe(29, None, 16, 16, 1, 1, None),
+    lambda l: l.fk20_fp_mul_probe(32, 16, 16, 16, 0, 1, None),
 ])
 def test_invalid_arguments_rejected(call):
     assert call(L.lib()) == -1  # FK20_EINVAL
