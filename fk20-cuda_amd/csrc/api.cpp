@@ -116,6 +116,11 @@ void fk20::build_host_tables(DevTables &T) {
         fs_from_mont(std, acc);
         memcpy(T.inv2k_mont[j], acc.l, 32);
         memcpy(T.inv2k_std[j], std.l, 32);
+        fs t;
+        fs_set(t, T.tw_mont[384]);
+        fs_mul(t, t, acc);
+        fs_from_mont(std, t);
+        memcpy(T.inv2k_w384_std[j], std.l, 32);
         fs_mul(acc, acc, half);
     }
 }
@@ -253,10 +258,12 @@ static int g1_transform(fk20_g1p_t *out, const fk20_g1p_t *in, unsigned count, u
     int rc = device_state(&st);
     if (rc) return rc;
     void *D;
-    rc = device_scratch(st, (size_t)count * n * 144, &D);
+    const size_t d_bytes = (size_t)count * n * 144;
+    rc = device_scratch(st, d_bytes + (size_t)count * 4, &D);
     if (rc) return rc;
+    uint32_t *flags = reinterpret_cast<uint32_t *>(reinterpret_cast<char *>(D) + d_bytes);
     return hip_status(launch_plain(inv ? PLAIN_G1_IFT : PLAIN_G1_FFT, W(out), W(in), nullptr, count, log2u(n),
-                                   reinterpret_cast<uint32_t *>(D), nullptr, nullptr, st->tables, st->prog,
+                                   reinterpret_cast<uint32_t *>(D), nullptr, nullptr, flags, st->tables, st->prog,
                                    S(stream)));
 }
 
@@ -301,19 +308,22 @@ int fk20_hext_fft2h_fft(fk20_g1p_t *h_fft, const fk20_g1p_t *hext_fft, unsigned 
     int rc = device_state(&st);
     if (rc) return rc;
     void *D;
-    rc = device_scratch(st, (size_t)rows * n2 * 144, &D);
+    const size_t d_bytes = (size_t)rows * n2 * 144;
+    rc = device_scratch(st, d_bytes + (size_t)rows * 4, &D);
     if (rc) return rc;
+    uint32_t *flags = reinterpret_cast<uint32_t *>(reinterpret_cast<char *>(D) + d_bytes);
     return hip_status(launch_plain(PLAIN_HEXT_FFT2H_FFT, W(h_fft), W(hext_fft), nullptr, rows, log2u(n2),
-                                   reinterpret_cast<uint32_t *>(D), nullptr, nullptr, st->tables, st->prog,
+                                   reinterpret_cast<uint32_t *>(D), nullptr, nullptr, flags, st->tables, st->prog,
                                    S(stream)));
 }
 
 // fk20_poly2h_fft.cu:47-80: toeplitz -> NTT (16 per row) -> MSM -> IFFT -> zero -> FFT, with
 // the 1/n2 of the IFFT moved onto the NTT output (a field multiplication instead of a
 // scalar multiplication per point) and the transforms column-batched over the rows.
-// Workspace: tcT, D, the packed bases and the MSM digits, in that order.
+// Workspace: tcT, D, the packed bases, the subgroup flag of the bases (one 256-byte slot)
+// and the MSM digits, in that order.
 static size_t poly2h_scratch_bytes(unsigned rows, unsigned n2) {
-    return (size_t)rows * 16 * n2 * 32 + (size_t)rows * n2 * 144 + (size_t)16 * n2 * 144 +
+    return (size_t)rows * 16 * n2 * 32 + (size_t)rows * n2 * 144 + (size_t)16 * n2 * 144 + 256 +
            msm_plain_digit_bytes(rows, n2);
 }
 
@@ -326,11 +336,14 @@ static int poly2h_fft_in(void *scratch, DeviceState *st, fk20_g1p_t *h_fft, cons
     uint32_t *tcT = reinterpret_cast<uint32_t *>(base);
     uint32_t *D = reinterpret_cast<uint32_t *>(base + tc_bytes);
     uint32_t *xpk = reinterpret_cast<uint32_t *>(base + tc_bytes + d_bytes);
-    int8_t *digits = reinterpret_cast<int8_t *>(base + tc_bytes + d_bytes + x_bytes);
+    uint32_t *flag = reinterpret_cast<uint32_t *>(base + tc_bytes + d_bytes + x_bytes);
+    int8_t *digits = reinterpret_cast<int8_t *>(base + tc_bytes + d_bytes + x_bytes + 256);
     hipStream_t s = S(stream);
+    // one flag for all rows: a base outside G1 can put every row's MSM output outside it
+    FK20_TRY(launch_subgroup_groups(flag, W(xext_fft), (size_t)16 * n2, (size_t)16 * n2, s));
     FK20_TRY(launch_std_to_packed(xpk, W(xext_fft), (size_t)16 * n2, s));
     return hip_status(launch_plain(PLAIN_POLY2H_FFT, W(h_fft), W(poly), xpk, rows, log2u(n2), D, tcT, digits,
-                                   st->tables, st->prog, s));
+                                   flag, st->tables, st->prog, s));
 }
 
 int fk20_poly2h_fft(fk20_g1p_t *h_fft, const fk20_fr_t *poly, const fk20_g1p_t *xext_fft, unsigned rows,
@@ -369,12 +382,15 @@ int fk20_setup2xext_fft(fk20_g1p_t *xext_fft, const fk20_g1p_t *setup, unsigned 
     int rc = device_state(&st);
     if (rc) return rc;
     void *D;
-    rc = device_scratch(st, (size_t)16 * (N / 8) * 144, &D);
+    const size_t d_bytes = (size_t)16 * (N / 8) * 144;
+    rc = device_scratch(st, d_bytes + 16 * 4, &D);
     if (rc) return rc;
+    uint32_t *flags = reinterpret_cast<uint32_t *>(reinterpret_cast<char *>(D) + d_bytes);
     hipStream_t s = S(stream);
     FK20_TRY(launch_setup_gather(W(xext_fft), W(setup), N, s));
     return hip_status(launch_plain(PLAIN_G1_FFT, W(xext_fft), W(xext_fft), nullptr, 16, log2u(N / 8),
-                                   reinterpret_cast<uint32_t *>(D), nullptr, nullptr, st->tables, st->prog, s));
+                                   reinterpret_cast<uint32_t *>(D), nullptr, nullptr, flags, st->tables, st->prog,
+                                   s));
 }
 
 // ---------------------------------------------------------------- helpers

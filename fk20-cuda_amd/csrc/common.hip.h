@@ -14,6 +14,7 @@ struct DevTables {
     uint32_t tw_std[513][8];      // w^i, standard form (scalar-multiplier input)
     uint32_t inv2k_mont[10][8];   // 2^-j mod r, Montgomery form, j = 0..9
     uint32_t inv2k_std[10][8];    // 2^-j mod r, standard form
+    uint32_t inv2k_w384_std[10][8];  // 2^-j w^384 = 2^-j w^-128 mod r, standard form
 };
 
 // Reference ABI element sizes in 32-bit words

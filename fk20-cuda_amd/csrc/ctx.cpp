@@ -500,9 +500,12 @@ int fk20_ctx_set_graph(fk20_ctx_t *c, int enable) {
 static hipError_t grow_work(fk20_ctx *c, size_t bytes);
 
 // Window tables of nbases points (standard-form g1p_t, device), each first multiplied by the
-// standard-form scalar `scale` (device, 8 words).  Fills st.table / st.inf.
+// standard-form scalar `scale` (device, 8 words).  Fills st.table / st.inf.  *outside: some
+// base lies outside the order-r subgroup G1; *small_order: a base of small order outside G1
+// gave an infinity entry, which the affine tables cannot hold (nothing is kept then).
 static hipError_t build_tables(Setup &st, const fk20_g1p_t *points, unsigned nbases, const uint32_t *scale,
-                               hipStream_t s) {
+                               hipStream_t s, bool *outside, bool *small_order) {
+    *outside = *small_order = false;
     const unsigned H = 1u << (st.win - 1);
     uint32_t *xs = nullptr, *zt = nullptr;
     // bases per build launch: one thread per (base, window), batch-inversion scratch bounded
@@ -512,17 +515,29 @@ static hipError_t build_tables(Setup &st, const fk20_g1p_t *points, unsigned nba
     unsigned chunk = (unsigned)std::max<size_t>(1, ((size_t)4 << 30) / zt_per_base);
     if (chunk > nbases) chunk = nbases;
     hipError_t e = hipMalloc(&st.table, (size_t)nbases * per_base);
-    if (e == hipSuccess) e = hipMalloc(&st.inf, nbases * 4);
+    // inf[0 .. nbases), then two words: outside G1, table-build error
+    if (e == hipSuccess) e = hipMalloc(&st.inf, (size_t)nbases * 4 + 8);
     if (e == hipSuccess) e = hipMalloc(&xs, (size_t)nbases * 144);
     if (e == hipSuccess) e = hipMalloc(&zt, table_build_scratch_bytes(chunk, st.win, st.nwin));
+    uint32_t *words = st.inf ? st.inf + nbases : nullptr;
+    if (e == hipSuccess) e = hipMemsetAsync(words, 0, 8, s);
+    if (e == hipSuccess)
+        e = launch_subgroup_groups(words, reinterpret_cast<const uint32_t *>(points), nbases, nbases, s);
     if (e == hipSuccess)
         e = launch_scale_bases(xs, st.inf, reinterpret_cast<const uint32_t *>(points), nbases, scale, s);
     for (unsigned b0 = 0; e == hipSuccess && b0 < nbases; b0 += chunk)
-        e = launch_table_build(st.table, zt, xs, st.inf, b0, std::min(chunk, nbases - b0), st.win, st.nwin, s);
+        e = launch_table_build(st.table, zt, xs, st.inf, words + 1, b0, std::min(chunk, nbases - b0), st.win,
+                               st.nwin, s);
+    uint32_t host_words[2] = {0, 0};
+    if (e == hipSuccess) e = hipMemcpyAsync(host_words, words, 8, hipMemcpyDeviceToHost, s);
     if (e == hipSuccess) e = hipStreamSynchronize(s);
+    if (e == hipSuccess) {
+        *outside = host_words[0] != 0;
+        *small_order = host_words[1] != 0;
+    }
     if (xs) (void)hipFree(xs);
     if (zt) (void)hipFree(zt);
-    if (e != hipSuccess) {
+    if (e != hipSuccess || *small_order) {
         if (st.table) (void)hipFree(st.table);
         if (st.inf) (void)hipFree(st.inf);
         st.table = st.inf = nullptr;
@@ -540,9 +555,17 @@ int fk20_setup_load(fk20_ctx_t *c, const fk20_g1p_t *xext_fft, unsigned N, unsig
     st.win = window;
     st.nwin = nwin_of(window);
     const unsigned logn2 = st.n2 == 128 ? 7 : st.n2 == 256 ? 8 : 9;
-    // the bases carry the IFFT's 1/n2
-    hipError_t e = build_tables(st, xext_fft, 16 * st.n2, c->T->inv2k_std[logn2], reinterpret_cast<hipStream_t>(stream));
+    // the bases carry the IFFT's 1/n2.  Every base must lie in G1: outside it the result
+    // depends on the exact integer multipliers (fk20_poly2h_fft follows the reference's).
+    bool outside, small_order;
+    hipError_t e = build_tables(st, xext_fft, 16 * st.n2, c->T->inv2k_std[logn2],
+                                reinterpret_cast<hipStream_t>(stream), &outside, &small_order);
     if (e != hipSuccess) return status_of(e);
+    if (outside) {
+        (void)hipFree(st.table);
+        (void)hipFree(st.inf);
+    }
+    if (outside || small_order) return FK20_EINVAL;
     st.live = true;
     c->setups.push_back(st);
     *setup_id = (int)c->setups.size() - 1;
@@ -560,8 +583,11 @@ int fk20_commit_setup_load(fk20_ctx_t *c, const fk20_g1p_t *srs, unsigned N, uns
     st.win = window;
     st.nwin = nwin_of(window);
     // tw_std[0] = w^0 = 1 (standard form): the SRS points are taken as they are
-    hipError_t e = build_tables(st, srs, N, c->T->tw_std[0], reinterpret_cast<hipStream_t>(stream));
+    bool outside, small_order;  // commitments multiply each point by its coefficient only: any point will do
+    hipError_t e = build_tables(st, srs, N, c->T->tw_std[0], reinterpret_cast<hipStream_t>(stream), &outside,
+                                &small_order);
     if (e != hipSuccess) return status_of(e);
+    if (small_order) return FK20_EINVAL;
     st.live = true;
     c->setups.push_back(st);
     *setup_id = (int)c->setups.size() - 1;

@@ -33,6 +33,11 @@ void free_programs(Programs &P);
 
 hipError_t launch_scale_bases(uint32_t *xs, uint32_t *inf, const uint32_t *xext, size_t count,
                               const uint32_t *scale, hipStream_t s);
+// flags[g] = 1 if any of the points pts[g * per_group .. (g + 1) * per_group) (standard-form
+// g1p_t, count in all) lies outside the order-r subgroup, else 0 (the flags of the groups are
+// cleared first); groups = ceil(count / per_group).
+hipError_t launch_subgroup_groups(uint32_t *flags, const uint32_t *pts, size_t count, size_t per_group,
+                                  hipStream_t s);
 // Window tables of bases [base0, base0 + nb): zt is workspace of table_build_scratch_bytes(nb,
 // win_bits, nwin) bytes.
 static constexpr unsigned TABLE_BLK = 256;  // entries per batch inversion
@@ -40,8 +45,11 @@ inline size_t table_build_scratch_bytes(unsigned nb, unsigned win_bits, unsigned
     const unsigned H = 1u << (win_bits - 1);
     return (size_t)nb * nwin * (H < TABLE_BLK ? H : TABLE_BLK) * 96;
 }
+// *err |= 1 when an entry d * 2^(w win) * base is infinity (a base of small order outside G1:
+// an affine table cannot hold it).
 hipError_t launch_table_build(uint32_t *table, uint32_t *zt, const uint32_t *xs, const uint32_t *inf,
-                              unsigned base0, unsigned nb, unsigned win_bits, unsigned nwin, hipStream_t s);
+                              uint32_t *err, unsigned base0, unsigned nb, unsigned win_bits, unsigned nwin,
+                              hipStream_t s);
 // KZG commitments of `rows` polynomials of length N over an SRS setup's window tables.
 size_t commit_workspace_bytes(unsigned rows, unsigned N);
 hipError_t launch_commit(uint32_t *out, const uint32_t *poly, const uint32_t *table, const uint32_t *inf,
@@ -60,6 +68,10 @@ struct Item {
     uint32_t out_format;    // FK20_OUT_* (bit 0: data-availability order, bit 1: compressed)
     uint32_t scale_tc;      // 1: the NTT output is scaled by 1/n2 (setups without folded 1/n2)
     int32_t stage;          // set per launch by launch_batch
+    // rowflag[row * flag_stride] != 0: the row holds a point outside the order-r subgroup and
+    // is recomputed by launch_ref_rows, so the output kernel leaves it alone (nullptr: none)
+    const uint32_t *rowflag;
+    uint32_t flag_stride;
 };
 struct Launch {
     Item item[MAX_ITEMS];
@@ -139,8 +151,11 @@ hipError_t launch_batch(const Launch &L, const DevTables *T, const uint32_t *ops
 //                         Montgomery bases), then as above            (fk20_poly2h_fft)
 // out may alias in: every input is read into D before the first output is written.
 enum PlainKind { PLAIN_G1_FFT = 0, PLAIN_G1_IFT = 1, PLAIN_HEXT_FFT2H_FFT = 2, PLAIN_POLY2H_FFT = 3 };
+// flags: `rows` words of workspace (one word for PLAIN_POLY2H_FFT, set by the caller with
+// launch_subgroup_groups over the bases).  Rows holding a point outside G1 are recomputed in
+// the reference's order of operations (launch_ref_rows).
 hipError_t launch_plain(PlainKind kind, uint32_t *out, const uint32_t *in, const uint32_t *xpk, unsigned rows,
-                        unsigned logn, uint32_t *D, uint32_t *tcT, int8_t *digits, const DevTables *T,
-                        const Programs &P, hipStream_t s);
+                        unsigned logn, uint32_t *D, uint32_t *tcT, int8_t *digits, uint32_t *flags,
+                        const DevTables *T, const Programs &P, hipStream_t s);
 
 }  // namespace fk20

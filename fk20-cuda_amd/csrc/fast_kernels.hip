@@ -65,6 +65,117 @@ __global__ void __launch_bounds__(64) k_scale_bases(uint32_t *xs, uint32_t *inf,
     pt_store(xs + b * 36, p);
 }
 
+// ---------------------------------------------------------------- subgroup membership
+
+__global__ void k_fill_u32(uint32_t *p, size_t n, uint32_t v) {
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
+// flags[i / per_group] |= 1 when pts[i] lies outside the order-r subgroup.
+__global__ void __launch_bounds__(64) k_subgroup_groups(uint32_t *flags, const uint32_t *pts, size_t count,
+                                                        size_t per_group) {
+    const size_t i = (size_t)blockIdx.x * 64 + threadIdx.x;
+    if (i >= count) return;
+    uint32_t w[36];
+    ld_words(w, pts + i * G1P_WORDS, 9);
+    g1p p;
+    g1_load_std(p, w);
+    if (!g1_in_subgroup(p)) atomicOr(flags + i / per_group, 1u);
+}
+
+// ---------------------------------------------------------------- rows outside G1
+//
+// On a point P outside the order-r subgroup, [a]P and [b]P differ whenever a != b as integers,
+// even for a = b (mod r): a transform of such points is defined by its exact sequence of
+// integer multipliers, not by the linear map mod r that the column-batched kernels compute.
+// A row that holds such a point is recomputed here in the reference's order of operations:
+// radix-2 decimation in time over bit-reversed positions, twiddle w^e applied to the right
+// point of a butterfly as the canonical integer w^e mod r (g1p_fft.cu:25-170); the inverse
+// with the butterflies transposed, w^-e applied after them and 2^-logn (and 2^-logn w^-128)
+// in stage 1 (g1p_fft.cu:180-330); hext_fft -> h_fft as inverse, odd positions cleared,
+// forward (fk20_hext_fft2h_fft.cu:27-70).  Products are double-and-add over complete
+// formulas (g1_mul_std, as g1p_mul.cu:18).  One workgroup of 256 threads per row; the row's n
+// points live in `work` (n * 36 words per row, row-contiguous); barriers order the stages.
+
+__device__ __forceinline__ void ref_butterfly(uint32_t *Wr, unsigned s, unsigned t, bool inverse, unsigned logn,
+                                              const DevTables *T) {
+    const unsigned h = 1u << s, j = t & (h - 1);
+    const unsigned l = ((t >> s) << (s + 1)) + j, r = l + h;
+    const unsigned e = j << (8 - s);  // twiddle w^e of the order-512 root w
+    g1p a, b;
+    pt_load(a, Wr + (size_t)l * 36);
+    pt_load(b, Wr + (size_t)r * 36);
+    if (!inverse) {
+        if (e) g1_mul_std(b, b, T->tw_std[e]);
+        g1_addsub(a, b);
+    } else {
+        g1_addsub(a, b);
+        if (s == 1) {
+            g1_mul_std(a, a, T->inv2k_std[logn]);
+            g1_mul_std(b, b, j ? T->inv2k_w384_std[logn] : T->inv2k_std[logn]);
+        } else if (s > 1 && e) {
+            g1_mul_std(b, b, T->tw_std[512 - e]);
+        }
+    }
+    pt_store(Wr + (size_t)l * 36, a);
+    pt_store(Wr + (size_t)r * 36, b);
+}
+
+// kind: 0 forward (in natural order -> out natural order), 1 inverse, 2 hext_fft -> h_fft.
+// in_D != nullptr: the input is D[bitrev(j)][row] (packed Montgomery, the MSM's output layout)
+// instead of in[row][j].
+__global__ void __launch_bounds__(256) k_ref_rows(unsigned kind, uint32_t *out, const uint32_t *in,
+                                                 const uint32_t *in_D, unsigned rows, unsigned logn, uint32_t *work,
+                                                 const uint32_t *flags, unsigned flag_stride, const DevTables *T) {
+    const unsigned row = blockIdx.x;
+    if (!flags[(size_t)row * flag_stride]) return;  // uniform over the workgroup
+    const unsigned n = 1u << logn, t = threadIdx.x;
+    uint32_t *Wr = work + (size_t)row * n * 36;
+    for (unsigned j = t; j < n; j += 256) {
+        g1p p;
+        if (in_D) {
+            pt_load(p, in_D + ((size_t)bitrev(j, logn) * rows + row) * 36);
+        } else {
+            uint32_t w[36];
+            ld_words(w, in + ((size_t)row * n + j) * G1P_WORDS, 9);
+            g1_load_std(p, w);
+        }
+        pt_store(Wr + (size_t)(kind == 0 ? bitrev(j, logn) : j) * 36, p);
+    }
+    __syncthreads();
+    if (kind != 0) {
+#pragma unroll 1
+        for (int s = (int)logn - 1; s >= 0; s--) {
+            if (t < n / 2) ref_butterfly(Wr, (unsigned)s, t, true, logn, T);
+            __syncthreads();
+        }
+        if (kind == 2) {
+            // the upper half of the inverse's natural-order output sits at the odd positions
+            for (unsigned j = 2 * t + 1; j < n; j += 512) {
+                g1p z;
+                g1_inf(z);
+                pt_store(Wr + (size_t)j * 36, z);
+            }
+            __syncthreads();
+        }
+    }
+    if (kind != 1) {
+#pragma unroll 1
+        for (unsigned s = 0; s < logn; s++) {
+            if (t < n / 2) ref_butterfly(Wr, s, t, false, logn, T);
+            __syncthreads();
+        }
+    }
+    for (unsigned j = t; j < n; j += 256) {
+        g1p p;
+        pt_load(p, Wr + (size_t)(kind == 1 ? bitrev(j, logn) : j) * 36);
+        uint32_t w[36];
+        g1_store_std(w, p);
+        st_words(out + ((size_t)row * n + j) * G1P_WORDS, w, 9);
+    }
+}
+
 // ---------------------------------------------------------------- setup: window tables
 
 // One thread per (base, window) of the launch's bases [base0, base0 + nb): entries d * B' for
@@ -74,8 +185,8 @@ __global__ void __launch_bounds__(64) k_scale_bases(uint32_t *xs, uint32_t *inf,
 // 64 lanes of a wave touch consecutive words; it is sized by the launch, not by H, so every
 // (base, window) of a setup fits in one or two launches.
 __global__ void __launch_bounds__(64, 3) k_table_build(uint32_t *table, uint32_t *zt, const uint32_t *xs,
-                                                    const uint32_t *inf, unsigned base0, unsigned nb,
-                                                    unsigned win_bits, unsigned nwin) {
+                                                    const uint32_t *inf, uint32_t *err, unsigned base0,
+                                                    unsigned nb, unsigned win_bits, unsigned nwin) {
     const unsigned nthreads = nb * nwin;
     const unsigned t = blockIdx.x * 64 + threadIdx.x;
     if (t >= nthreads) return;
@@ -90,7 +201,8 @@ __global__ void __launch_bounds__(64, 3) k_table_build(uint32_t *table, uint32_t
         return;
     }
     // B' = 2^(w win) B in affine form, so each entry is one mixed addition of B' (B' is not
-    // infinity: the base is not, and its order r is odd)
+    // infinity: the base is not, and E(Fp) has odd order).  An entry d B' is infinity only
+    // for a base of small order outside G1; the block's inversion then fails and *err says so.
     g1p E;
     fe bx, by;
     {
@@ -120,6 +232,7 @@ __global__ void __launch_bounds__(64, 3) k_table_build(uint32_t *table, uint32_t
         }
         fe inv;
         fe_canonical(prefix, prefix);
+        if (fe_is_zero_canonical(prefix)) atomicOr(err, 1u);
         fe_inv(inv, prefix);
 #pragma unroll 1
         for (int i = (int)blk - 1; i >= 0; i--) {
@@ -194,7 +307,8 @@ __global__ void __launch_bounds__(256) k_toeplitz_fft_multi(Launch L, const DevT
     if (t < half) {
         for (unsigned e = t; e < n; e += half) {
             fs a = lds[e];
-            if (it.scale_tc) fs_mul(a, a, scale);
+            // the reference's order (k_ref_rows) applies 1/n2 inside the inverse transform
+            if (it.scale_tc && !(it.rowflag && it.rowflag[(size_t)row * it.flag_stride])) fs_mul(a, a, scale);
             fs_from_mont(a, a);
             st_words(it.tcT + (((size_t)e * 16 + i) * it.rows + row) * FR_WORDS, a.l, 2);
         }
@@ -384,6 +498,7 @@ __global__ void __launch_bounds__(64) k_output_multi(Launch L) {
     const unsigned pos = local / rg;
     const unsigned row = (local % rg) * 64 + threadIdx.x;
     if (row >= rows) return;
+    if (it.rowflag && it.rowflag[(size_t)row * it.flag_stride]) return;  // written by k_ref_rows
     g1p p;
     pt_load(p, it.D + ((size_t)pos * rows + row) * 36);
     const unsigned fmt = it.out_format;
@@ -401,6 +516,17 @@ __global__ void __launch_bounds__(64) k_output_multi(Launch L) {
 
 // ---------------------------------------------------------------- launchers
 
+hipError_t launch_subgroup_groups(uint32_t *flags, const uint32_t *pts, size_t count, size_t per_group,
+                                  hipStream_t s) {
+    if (!count || !per_group) return hipSuccess;
+    const size_t groups = (count + per_group - 1) / per_group;
+    // a fill kernel, not hipMemsetAsync: callers may capture these launches into a graph
+    hipLaunchKernelGGL(k_fill_u32, dim3((unsigned)((groups + 255) / 256)), dim3(256), 0, s, flags, groups, 0u);
+    hipLaunchKernelGGL(k_subgroup_groups, dim3((unsigned)((count + 63) / 64)), dim3(64), 0, s, flags, pts, count,
+                       per_group);
+    return hipGetLastError();
+}
+
 hipError_t launch_scale_bases(uint32_t *xs, uint32_t *inf, const uint32_t *xext, size_t count,
                               const uint32_t *scale, hipStream_t s) {
     hipLaunchKernelGGL(k_scale_bases, dim3((unsigned)((count + 63) / 64)), dim3(64), 0, s, xs, inf, xext, count,
@@ -409,10 +535,11 @@ hipError_t launch_scale_bases(uint32_t *xs, uint32_t *inf, const uint32_t *xext,
 }
 
 hipError_t launch_table_build(uint32_t *table, uint32_t *zt, const uint32_t *xs, const uint32_t *inf,
-                              unsigned base0, unsigned nb, unsigned win_bits, unsigned nwin, hipStream_t s) {
+                              uint32_t *err, unsigned base0, unsigned nb, unsigned win_bits, unsigned nwin,
+                              hipStream_t s) {
     const unsigned threads = nb * nwin;
-    hipLaunchKernelGGL(k_table_build, dim3((threads + 63) / 64), dim3(64), 0, s, table, zt, xs, inf, base0, nb,
-                       win_bits, nwin);
+    hipLaunchKernelGGL(k_table_build, dim3((threads + 63) / 64), dim3(64), 0, s, table, zt, xs, inf, err, base0,
+                       nb, win_bits, nwin);
     return hipGetLastError();
 }
 
@@ -582,9 +709,16 @@ hipError_t launch_batch(const Launch &base, const DevTables *T, const uint32_t *
 // stages 0 .. L-1 (the first likewise), and the output kernel writes natural order from the
 // DIF's bit-reversed positions (format 0) or position by position (format 1).
 hipError_t launch_plain(PlainKind kind, uint32_t *out, const uint32_t *in, const uint32_t *xpk, unsigned rows,
-                        unsigned logn, uint32_t *D, uint32_t *tcT, int8_t *digits, const DevTables *T,
-                        const Programs &P, hipStream_t s) {
+                        unsigned logn, uint32_t *D, uint32_t *tcT, int8_t *digits, uint32_t *flags,
+                        const DevTables *T, const Programs &P, hipStream_t s) {
     if (!rows) return hipSuccess;
+    // rows holding a point outside G1 are recomputed by k_ref_rows: the transform kinds check
+    // their inputs here, one flag per row; PLAIN_POLY2H_FFT's caller has set flags[0] from the
+    // bases, which every row's MSM output combines
+    if (kind != PLAIN_POLY2H_FFT) {
+        hipError_t e = launch_subgroup_groups(flags, in, (size_t)rows << logn, (size_t)1 << logn, s);
+        if (e != hipSuccess) return e;
+    }
     Launch L;
     memset(&L, 0, sizeof L);
     L.nitems = 1;
@@ -597,6 +731,8 @@ hipError_t launch_plain(PlainKind kind, uint32_t *out, const uint32_t *in, const
     it.N = 8u << logn;
     it.logn2 = logn;
     it.scale_tc = 1;
+    it.rowflag = flags;
+    it.flag_stride = kind == PLAIN_POLY2H_FFT ? 0 : 1;
     const unsigned rg = (rows + 63) / 64, n = 1u << logn;
     unsigned per[MAX_ITEMS] = {};
     const uint32_t *scale = P.ops + P.scale_off[logn];
@@ -639,6 +775,10 @@ hipError_t launch_plain(PlainKind kind, uint32_t *out, const uint32_t *in, const
             hipLaunchKernelGGL(k_toeplitz_fft_multi, dim3(set_blocks(L, per)), dim3(256), 0, s, L, T);
             hipError_t e = launch_msm_plain_T(D, tcT, xpk, digits, rows, n, s);
             if (e != hipSuccess) return e;
+            // rows of a setup outside G1 take the reference's order from the MSM output on,
+            // with tcT (no longer read) as their workspace
+            hipLaunchKernelGGL(k_ref_rows, dim3(rows), dim3(256), 0, s, 2u, out, (const uint32_t *)nullptr,
+                               (const uint32_t *)D, rows, logn, tcT, (const uint32_t *)flags, 0u, T);
         }
         for (unsigned t = 0; t + 1 < logn; t++) stage(K.dit, (int)t);
         stage(K.middle, 0);
@@ -647,6 +787,12 @@ hipError_t launch_plain(PlainKind kind, uint32_t *out, const uint32_t *in, const
     }
     per[0] = n * rg;
     hipLaunchKernelGGL(k_output_multi, dim3(set_blocks(L, per)), dim3(64), 0, s, L);
+    // flagged rows of the transform kinds: their input rows are intact (the output kernel
+    // skipped them, so an aliased output has not overwritten them) and D is free as workspace
+    if (kind != PLAIN_POLY2H_FFT)
+        hipLaunchKernelGGL(k_ref_rows, dim3(rows), dim3(256), 0, s,
+                           kind == PLAIN_G1_FFT ? 0u : kind == PLAIN_G1_IFT ? 1u : 2u, out, in,
+                           (const uint32_t *)nullptr, rows, logn, D, (const uint32_t *)flags, 1u, T);
     return hipGetLastError();
 }
 

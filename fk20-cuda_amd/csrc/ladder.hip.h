@@ -25,7 +25,8 @@
 // accumulator before each addition is a P and the entry b P with a, b known scalars; the host
 // checks every program (ctx.cpp) for a != 0 and a != +-b mod r, so no exceptional case
 // occurs.  An input at infinity (Z = 0) is detected and returned as infinity.  Inputs outside
-// the order-r subgroup are not supported (the GLV split assumes phi(P) = lambda P).
+// the order-r subgroup are not handled here (the GLV split assumes phi(P) = lambda P): the
+// transforms find such rows with g1_in_subgroup below and recompute them on another path.
 #pragma once
 #include "g1.hip.h"
 
@@ -120,6 +121,33 @@ FK20_HD void tw_mul(g1p &x, const uint32_t *ops) {
     fe_mul(x.x, acc.x, z);
     x.y = acc.y;
     if (inf) g1_inf(x);
+}
+
+// Whether p lies in the order-r subgroup G1 (infinity included): phi^2(p) = (beta^2 x, y)
+// acts on G1 as lambda^2 = -z^2, and a point of E(Fp) satisfies phi^2(p) + [z^2] p = 0 only
+// if it lies in G1 (Scott, "A note on group membership tests for G1, G2 and GT on BLS
+// pairing-friendly curves", 2021).  [z^2] p = [|z|]([|z|] p) by double-and-add over
+// |z| = 0xd201000000010000 (63 doublings and 5 additions per round), complete formulas.
+FK20_HD bool g1_in_subgroup(const g1p &p) {
+    constexpr uint64_t ZABS = 0xd201000000010000ull;
+    g1p q = p;
+#pragma unroll 1
+    for (int round = 0; round < 2; round++) {
+        const g1p base = q;
+#pragma unroll 1
+        for (int b = 62; b >= 0; b--) {
+            g1_dbl(q, q);
+            if ((ZABS >> b) & 1) g1_add(q, q, base);
+        }
+    }
+    g1p f = p;
+    fe beta2;
+    fe_set(beta2, FP_BETA2);
+    fe_mul(f.x, f.x, beta2);
+    g1_add(q, q, f);
+    fe zc;
+    fe_canonical(zc, q.z);
+    return fe_is_zero_canonical(zc);
 }
 
 }  // namespace fk20

@@ -32,9 +32,11 @@ class Prover:
         buf = xext_fft if isinstance(xext_fft, DeviceBuffer) else DeviceBuffer.from_array(
             np.ascontiguousarray(xext_fft, dtype=np.uint64))
         sid = ctypes.c_int()
-        call("fk20_setup_load", self.ctx, buf.ptr, N, window, ctypes.byref(sid), stream)
-        if buf is not xext_fft:
-            buf.free()
+        try:
+            call("fk20_setup_load", self.ctx, buf.ptr, N, window, ctypes.byref(sid), stream)
+        finally:
+            if buf is not xext_fft:
+                buf.free()
         self.setups[sid.value] = N
         return sid.value
 
@@ -43,9 +45,11 @@ class Prover:
         buf = srs if isinstance(srs, DeviceBuffer) else DeviceBuffer.from_array(
             np.ascontiguousarray(srs[:N], dtype=np.uint64))
         sid = ctypes.c_int()
-        call("fk20_commit_setup_load", self.ctx, buf.ptr, N, window, ctypes.byref(sid), stream)
-        if buf is not srs:
-            buf.free()
+        try:
+            call("fk20_commit_setup_load", self.ctx, buf.ptr, N, window, ctypes.byref(sid), stream)
+        finally:
+            if buf is not srs:
+                buf.free()
         self.setups[sid.value] = N
         return sid.value
 

@@ -120,3 +120,24 @@ def device_setup(N, tau):
     L.synchronize()
     srs.free()
     return xext
+
+
+def curve_points(count, seed):
+    """`count` points of E(Fp): y^2 = x^3 + 4 from random x with x^3 + 4 a square (p = 3 mod 4,
+    so y = (x^3 + 4)^((p+1)/4)), as standard-form g1p_t (count, 18) with z = 1.  Such points
+    lie outside the order-r subgroup G1 unless their cofactor part vanishes (probability
+    1/h, h ~ 2^126)."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    out = np.zeros((count, 18), dtype=np.uint64)
+    n = 0
+    while n < count:
+        x = int.from_bytes(rng.bytes(48), "little") % P_MOD
+        a = (x * x * x + 4) % P_MOD
+        y = pow(a, (P_MOD + 1) // 4, P_MOD)
+        if y * y % P_MOD != a:
+            continue
+        for c, v in enumerate((x, y, 1)):
+            for i in range(6):
+                out[n, 6 * c + i] = (v >> (64 * i)) & 0xFFFFFFFFFFFFFFFF
+        n += 1
+    return out

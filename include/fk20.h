@@ -14,7 +14,15 @@
  * Inputs may be any representative (fr: any 256-bit value, fp: any 384-bit value).
  * Outputs are standard form with every coordinate canonical (< p or < r); projective
  * outputs are not normalised (use fk20_g1p_to_affine for canonical affine points).
- * G1 inputs must be points of the order-r subgroup (all FK20 data are).
+ * G1 inputs may be any point of E(Fp), as in the reference.  On the order-r subgroup G1 (all
+ * FK20 data lie in it) a transform is a linear map mod r; on a point outside it [a]P and [b]P
+ * differ for a = b (mod r), so the transforms check their inputs and recompute a row holding
+ * such a point in the reference's exact order of operations (same butterflies, the canonical
+ * twiddles and 2^-9 as double-and-add multipliers), giving the reference's points.
+ * fk20_poly2h_fft does so from the MSM on, whose scalars are canonical here (the reference's
+ * may be other representatives mod r).  The batched prover (fk20_setup_load) takes setups in
+ * G1 only (FK20_EINVAL otherwise); fk20_commit_setup_load takes any point except one of small
+ * order, whose multiples reach infinity inside the affine window tables (FK20_EINVAL).
  *
  * Sizes: N is the polynomial length (1024, 2048 or 4096; the reference fixes 4096),
  * l = 16 is fixed, k = N/16, and n2 = 2k = N/8 is the FFT size (128, 256, 512).
@@ -199,7 +207,8 @@ size_t fk20_output_point_bytes(int format);
  * (8 to 13 bits): 16 * (N/8) * ceil(256/w) * 2^(w-1) * 96, e.g. 35 GiB at N=4096, w=12. */
 size_t fk20_setup_table_bytes(unsigned N, unsigned window);
 /* Builds the resident tables of xext_fft[16][N/8] (device memory, standard form);
- * synchronises `stream`.  The returned id names the setup in fk20_prove_rows. */
+ * synchronises `stream`.  The returned id names the setup in fk20_prove_rows.  FK20_EINVAL
+ * for a setup with a point outside G1. */
 int fk20_setup_load(fk20_ctx_t *ctx, const fk20_g1p_t *xext_fft, unsigned N, unsigned window,
                     int *setup_id, fk20_stream_t stream);
 int fk20_setup_unload(fk20_ctx_t *ctx, int setup_id);

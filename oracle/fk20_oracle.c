@@ -12,7 +12,8 @@
  *     from PRIMITIVE_ROOT = 5), kzg_proofs.py:198-205 (compute_proof_multi).
  *   - The CUDA stage split and layouts: fk20_poly2toeplitz_coefficients.cu:40-56,
  *     fk20_msm.cu:23-66, fk20_poly2h_fft.cu:27-80, fr_fft.cu:28-168 (natural-order
- *     DFT, bit-reversed load), g1p_fft.cu:25-170, g1p_mul.cu:18-66 (MSB-first
+ *     DFT, bit-reversed load), g1p_fft.cu:25-170 and 180-330 (forward and inverse
+ *     order of operations, which fixes the result on points outside G1), g1p_mul.cu:18-66 (MSB-first
  *     double-and-add over the raw 256-bit scalar), g1p.cu:95-115 (infinity and
  *     generator), g1a.cu:48-56 (affine conversion, affine infinity = (0,0)).
  *   - G1 formulas: complete projective formulas of Renes, Costello and Batina,
@@ -614,10 +615,63 @@ static u64 *fr_powers_std(int n, int inverse) {
     return tw;
 }
 
-/* In-place G1 DFT (g1p_fft.cu:25-170 structure; FK20Py/fft.py:39-55 semantics) */
-static void g1_fft_m(gp *a, int n, int inverse) {
+/* In-place G1 inverse DFT in the reference's order of operations (g1p_fft.cu:180-330):
+ * natural-order input; stages s = log2(n)-1 .. 0, each butterfly (l, r) <- (l + r, l - r)
+ * followed, for s >= 2, by r <- [w^-(t * step)] r; stage 1 instead multiplies l by 2^-log2(n)
+ * and r by 2^-log2(n) (t = 0) or 2^-log2(n) w^-(n/4) (t = 1) (fr_roots[513], fr_roots[514]);
+ * output in natural order from the bit-reversed positions.  On G1 this is the DFT with the
+ * inverse roots times n^-1 (FK20Py/fft.py:39-55); on points outside G1, where [a]P depends on
+ * the integer a and not only on a mod r, only this exact sequence of canonical multipliers
+ * gives the reference's result. */
+static void g1_ift_m(gp *a, int n) {
     int bits = log2i(n);
-    u64 *tw = fr_powers_std(n, inverse);
+    u64 *tw = fr_powers_std(n, 1);
+    u64 nn[4] = {(u64)n, 0, 0, 0}, ninv[4], ninv_std[4], ninv_w_std[4], t4[4];
+    to_mont(ninv, nn, &FR);
+    mont_inv(ninv, ninv, &FR);
+    from_mont(ninv_std, ninv, &FR);
+    to_mont(t4, tw + 4 * (n / 4), &FR);
+    mont_mul(t4, t4, ninv, &FR);
+    from_mont(ninv_w_std, t4, &FR);
+    for (int half = n / 2; half >= 1; half /= 2) {
+        int step = n / (2 * half);
+        for (int blk = 0; blk < n; blk += 2 * half) {
+            for (int t = 0; t < half; t++) {
+                gp *l = a + blk + t, *r = a + blk + t + half, nr, sum, dif;
+                gp_neg(&nr, r);
+                gp_add(&sum, l, r);
+                gp_add(&dif, l, &nr);
+                if (half == 2) {
+                    gp_mul(&sum, &sum, ninv_std);
+                    gp_mul(&dif, &dif, t ? ninv_w_std : ninv_std);
+                } else if (half > 2 && t) {
+                    gp_mul(&dif, &dif, tw + 4 * (t * step));
+                }
+                *l = sum;
+                *r = dif;
+            }
+        }
+    }
+    for (int i = 0; i < n; i++) {
+        int j = (int)bitrev((unsigned)i, bits);
+        if (j > i) {
+            gp t = a[i];
+            a[i] = a[j];
+            a[j] = t;
+        }
+    }
+    free(tw);
+}
+
+/* In-place G1 DFT (g1p_fft.cu:25-170 structure; FK20Py/fft.py:39-55 semantics); the inverse
+ * is g1_ift_m */
+static void g1_fft_m(gp *a, int n, int inverse) {
+    if (inverse) {
+        g1_ift_m(a, n);
+        return;
+    }
+    int bits = log2i(n);
+    u64 *tw = fr_powers_std(n, 0);
     for (int i = 0; i < n; i++) {
         int j = (int)bitrev((unsigned)i, bits);
         if (j > i) {
@@ -638,13 +692,6 @@ static void g1_fft_m(gp *a, int n, int inverse) {
                 gp_add(l, l, &x);
             }
         }
-    }
-    if (inverse) {
-        u64 nn[4] = {(u64)n, 0, 0, 0}, ninv[4], ninv_std[4];
-        to_mont(ninv, nn, &FR);
-        mont_inv(ninv, ninv, &FR);
-        from_mont(ninv_std, ninv, &FR);
-        for (int i = 0; i < n; i++) gp_mul(a + i, a + i, ninv_std);
     }
     free(tw);
 }

@@ -169,6 +169,13 @@ void host_tw_mul_w(uint64_t *r, const uint64_t *p, const uint32_t *ops, int w) {
     stp(r, a);
 }
 
+// The kernels' membership test of the order-r subgroup (ladder.hip.h).
+int host_g1_in_subgroup(const uint64_t *p) {
+    g1p a;
+    ldp(a, p);
+    return g1_in_subgroup(a) ? 1 : 0;
+}
+
 // Jacobian doubling / mixed addition, g1p_t slots holding Jacobian (X, Y, Z) and the affine
 // operand (x, y) in the first two coordinates; `reps` repeats the operation on its output.
 void host_g1j_dbl(uint64_t *r, const uint64_t *p, int reps) {

@@ -255,3 +255,47 @@ def test_ladder_other_widths(L, w):
         L.host_tw_mul_w(p64(out[n]), p64(pts[n]), ops.ctypes.data_as(ctypes.c_void_p), w)
     bad, first = O.match_affine(out, exp)
     assert bad == 0, "program %s differs" % (progs[first],)
+
+
+# cofactor of G1 in E(Fp): h = 3 * 11^2 * 10177^2 * 859267^2 * 52437899^2
+H_COF = 0x396c8c005555e1568c00aaab0000aaab
+
+
+def _mul_big(p, k):
+    """[k]p for an integer k of any size (the oracle's double-and-add takes 256 bits)."""
+    acc = O.g1p_inf()
+    chunks = []
+    while k:
+        chunks.append(k & ((1 << 128) - 1))
+        k >>= 128
+    for c in reversed(chunks):
+        acc = O.g1p_op("mul", acc, O.fr(1 << 128))
+        acc = O.g1p_op("add", acc, O.g1p_op("mul", p, O.fr(c)))
+    return acc
+
+
+def test_subgroup_membership(L):
+    """The kernels' G1 membership test (phi^2(P) + [z^2]P = 0) against the definition [r]P = 0:
+    G1 points, infinity, random points of E(Fp) (outside G1), their cofactor-cleared
+    multiples [h]P (in G1), points of small order (3, 11, 121) and G1 + small-order sums."""
+    G = O.g1p_gen()
+    pts = [G, O.g1p_inf(), O.g1p_op("mul", G, O.fr(R - 1))]
+    pts += [O.g1p_op("mul", G, s) for s in S.random_fr(3, 61)]
+    curve = list(S.curve_points(6, 62))
+    pts += curve
+    pts += [_mul_big(q, H_COF) for q in curve[:3]]
+    small = []
+    for q in curve[3:]:
+        rq = O.g1p_op("mul", q, O.fr(R))
+        for m in (3, 11, 121):
+            t = _mul_big(rq, H_COF // m)
+            if not O.g1p_is_inf(t):
+                small.append(t)
+    assert len(small) >= 3
+    pts += small
+    pts += [O.g1p_op("add", G, t) for t in small[:2]]
+    for p in pts:
+        want = O.g1p_is_inf(O.g1p_op("mul", p, O.fr(R)))
+        assert L.host_g1_in_subgroup(p64(np.ascontiguousarray(p))) == int(want)
+    # the random curve points really are outside G1 and their [h] multiples inside
+    assert not any(O.g1p_is_inf(O.g1p_op("mul", q, O.fr(R))) for q in curve)
