@@ -244,6 +244,42 @@ int status_of(hipError_t e) { return status_from_hip(e); }
 
 namespace {
 // Every program (w^+-i, then 2^-j) on the host, each checked, with its index table.
+// Ladder cost of an encoded program (the units of program_cost): its doublings, its mixed
+// additions after the first entry and its endomorphism products.
+int ops_cost(const uint32_t *ops) {
+    int dbl = 0, add = 0, phi = 0;
+    for (unsigned k = 0;; k++) {
+        dbl += (int)(ops[k] & TWOP_DBL_MASK);
+        if (ops[k] & TWOP_END) break;
+        add++;
+        phi += (ops[k] & (TWOP_PHI | TWOP_PHI2)) != 0;
+    }
+    return dbl * 2275 + std::max(add - 1, 0) * 3752 + phi * 392;
+}
+
+// The twiddle layers' item tables (index[TW_LAYER ..]): every (q, e) of layer_twiddle, per
+// size, stage and direction, sorted by decreasing cost of e's program (stable: ties keep the
+// closed-form order).
+bool layer_tables_host(const std::vector<uint32_t> &ops, std::vector<uint32_t> &index) {
+    for (unsigned L = 7; L <= 9; L++)
+        for (unsigned sd = 1; sd < L; sd += 2)
+            for (unsigned dir = 0; dir < 2; dir++) {
+                const unsigned cnt = layer_count(sd, L), off = layer_table_offset(L, sd, dir);
+                if (off + cnt > (unsigned)TW_LAYER_SIZE) return false;
+                std::vector<std::pair<int, uint32_t>> items;
+                for (unsigned k = 0; k < cnt; k++) {
+                    const LayerTw t = layer_twiddle(sd, L, k);
+                    items.push_back({-ops_cost(ops.data() + index[dir * 512 + t.e]), t.q | (t.e << 16)});
+                }
+                std::stable_sort(items.begin(), items.end(),
+                                 [](const std::pair<int, uint32_t> &a, const std::pair<int, uint32_t> &b) {
+                                     return a.first < b.first;
+                                 });
+                for (unsigned k = 0; k < cnt; k++) index[TW_LAYER + off + k] = items[k].second;
+            }
+    return true;
+}
+
 bool all_programs_host(std::vector<uint32_t> &ops, std::vector<uint32_t> &index, uint32_t *scale_off,
                        int W = TW_WNAF) {
     if (!build_programs_host(ops, index, W)) return false;
@@ -256,7 +292,7 @@ bool all_programs_host(std::vector<uint32_t> &ops, std::vector<uint32_t> &index,
         fs_set(expect, T.inv2k_mont[j]);
         if (!append_program(ops, T.inv2k_std[j], W, expect)) return false;
     }
-    return true;
+    return layer_tables_host(ops, index);
 }
 }  // namespace
 

@@ -12,7 +12,26 @@ static constexpr int TW_FWD = 0;
 static constexpr int TW_INV = 1;
 // index[TW_SCALE + j] = offset of the program of 2^-j (j < 10), the 1/n of the inverse transforms
 static constexpr int TW_SCALE = 2 * 512;
-static constexpr int TW_INDEX_SIZE = TW_SCALE + 10;
+// index[TW_LAYER + layer_table_offset(logn2, sd, dir) + k], k < layer_count(sd, logn2): the
+// k-th item of the twiddle layer after odd DIF stage sd, packed q | e << 16 (the LayerTw of
+// layer_twiddle), in order of decreasing ladder cost of its program in direction dir.  A
+// layer launch hands its items to the SIMDs in that order: the long ladders spread over the
+// chip first and the short ones (w^+-128, w^256) fill in after them, instead of two long
+// ladders meeting on one SIMD while others run short ones.
+static constexpr int TW_LAYER = TW_SCALE + 10;
+static constexpr int TW_LAYER_SIZE = 4096;  // 2 x (225 + 513 + 1281) items for n2 = 128, 256, 512
+static constexpr int TW_INDEX_SIZE = TW_LAYER + TW_LAYER_SIZE;
+
+FK20_HD unsigned layer_table_offset(unsigned logn2, unsigned sd, unsigned dir) {
+    unsigned off = 0;
+    for (unsigned L = 7; L <= 9; L++)
+        for (unsigned s = 1; s < L; s += 2)
+            for (unsigned d = 0; d < 2; d++) {
+                if (L == logn2 && s == sd && d == dir) return off;
+                off += layer_count(s, L);
+            }
+    return off;
+}
 
 // Twiddle programs of one device (ctx.cpp): ops and index in device memory; the scale
 // programs' offsets also on the host, so launchers can pass them as plain pointers.

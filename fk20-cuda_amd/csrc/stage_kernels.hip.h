@@ -101,7 +101,8 @@ FK20_STAGE_KERNEL(k_dif_multi, dif_butterfly, it.logn2 - 1 + (unsigned)it.stage)
 
 // The point-wise twiddle layer of odd DIF stage it.stage (radix-2^2 schedule): after that
 // stage's butterflies in the FFT (dir = TW_FWD), before the butterflies of the transposed DIT
-// stage in the IFFT (dir = TW_INV).  One wave per (nontrivial position, 64-row group).
+// stage in the IFFT (dir = TW_INV).  One wave per (nontrivial position, 64-row group), the
+// positions taken in the cost order of the index's layer table.
 template <int W, bool R22, int OCC>
 __global__ void __launch_bounds__(64, OCC) k_twiddle_multi(Launch L, const uint32_t *ops, const uint32_t *index,
                                                           unsigned dir) {
@@ -109,11 +110,11 @@ __global__ void __launch_bounds__(64, OCC) k_twiddle_multi(Launch L, const uint3
     const Item &it = L.item[find_item(L, blockIdx.x, local)];
     const unsigned rg = (it.rows + 63) / 64;
     const unsigned row = min((local % rg) * 64 + threadIdx.x, it.rows - 1);
-    const LayerTw t = layer_twiddle((unsigned)it.stage, it.logn2, local / rg);
-    uint32_t *p = it.D + ((size_t)t.q * it.rows + row) * 36;
+    const uint32_t item = index[TW_LAYER + layer_table_offset(it.logn2, (unsigned)it.stage, dir) + local / rg];
+    uint32_t *p = it.D + ((size_t)(item & 0xffffu) * it.rows + row) * 36;
     g1p x;
     pt_load(x, p);
-    tw_mul<W>(x, ops + index[dir * 512 + t.e]);
+    tw_mul<W>(x, ops + index[dir * 512 + (item >> 16)]);
     pt_store(p, x);
 }
 
