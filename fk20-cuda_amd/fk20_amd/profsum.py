@@ -104,6 +104,8 @@ def valu_summary(counter_csv, trace_csv):
         if k.startswith("__amd_rocclr") or "SQ_INSTS_VALU" not in c:
             continue
         m = {n: sum(v) / len(v) for n, v in c.items()}
+        if not m["SQ_INSTS_VALU"]:  # a launch too small to register (a fill of a few words)
+            continue
         t = sum(dur[k]) / len(dur[k]) if dur.get(k) else None
         cycles = m.get("GRBM_GUI_ACTIVE", 0) / XCDS
         e = {"dispatches": len(c["SQ_INSTS_VALU"]),
