@@ -72,16 +72,36 @@ __global__ void k_fill_u32(uint32_t *p, size_t n, uint32_t v) {
     if (i < n) p[i] = v;
 }
 
-// flags[i / per_group] |= 1 when pts[i] lies outside the order-r subgroup.
-__global__ void __launch_bounds__(64) k_subgroup_groups(uint32_t *flags, const uint32_t *pts, size_t count,
-                                                        size_t per_group) {
+// A lane's point in LDS, packed (36 words, stride 37 to spread the lanes over the banks).
+struct LdsSlot {
+    uint32_t *w;
+    __device__ void put(const g1p &p) {
+        uint32_t t[36];
+        g1_store_packed(t, p);
+#pragma unroll
+        for (int k = 0; k < 36; k++) w[k] = t[k];
+    }
+    __device__ void get(g1p &p) const {
+        uint32_t t[36];
+#pragma unroll
+        for (int k = 0; k < 36; k++) t[k] = w[k];
+        g1_load_packed(p, t);
+    }
+};
+
+// flags[i / per_group] |= 1 when pts[i] lies outside the order-r subgroup.  The test's two
+// saved points sit in LDS, which leaves room for two waves per SIMD without spills.
+__global__ void __launch_bounds__(64, 2) k_subgroup_groups(uint32_t *flags, const uint32_t *pts, size_t count,
+                                                           size_t per_group) {
+    __shared__ uint32_t slots[2][64 * 37];
     const size_t i = (size_t)blockIdx.x * 64 + threadIdx.x;
     if (i >= count) return;
     uint32_t w[36];
     ld_words(w, pts + i * G1P_WORDS, 9);
     g1p p;
     g1_load_std(p, w);
-    if (!g1_in_subgroup(p)) atomicOr(flags + i / per_group, 1u);
+    LdsSlot orig{slots[0] + threadIdx.x * 37}, base{slots[1] + threadIdx.x * 37};
+    if (!g1_in_subgroup(p, orig, base)) atomicOr(flags + i / per_group, 1u);
 }
 
 // ---------------------------------------------------------------- rows outside G1

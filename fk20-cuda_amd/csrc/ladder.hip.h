@@ -123,24 +123,40 @@ FK20_HD void tw_mul(g1p &x, const uint32_t *ops) {
     if (inf) g1_inf(x);
 }
 
+// A point held in registers (the host build; the kernel keeps its copies in LDS instead, to
+// leave the registers to the formulas).
+struct RegSlot {
+    g1p v;
+    FK20_HD void put(const g1p &p) { v = p; }
+    FK20_HD void get(g1p &p) const { p = v; }
+};
+
 // Whether p lies in the order-r subgroup G1 (infinity included): phi^2(p) = (beta^2 x, y)
 // acts on G1 as lambda^2 = -z^2, and a point of E(Fp) satisfies phi^2(p) + [z^2] p = 0 only
 // if it lies in G1 (Scott, "A note on group membership tests for G1, G2 and GT on BLS
 // pairing-friendly curves", 2021).  [z^2] p = [|z|]([|z|] p) by double-and-add over
 // |z| = 0xd201000000010000 (63 doublings and 5 additions per round), complete formulas.
-FK20_HD bool g1_in_subgroup(const g1p &p) {
+// `orig` and `base` hold p and each round's base while the doublings run.
+template <class Slot>
+FK20_HD bool g1_in_subgroup(const g1p &p, Slot &orig, Slot &base) {
     constexpr uint64_t ZABS = 0xd201000000010000ull;
+    orig.put(p);
     g1p q = p;
 #pragma unroll 1
     for (int round = 0; round < 2; round++) {
-        const g1p base = q;
+        base.put(q);
 #pragma unroll 1
         for (int b = 62; b >= 0; b--) {
             g1_dbl(q, q);
-            if ((ZABS >> b) & 1) g1_add(q, q, base);
+            if ((ZABS >> b) & 1) {
+                g1p t;
+                base.get(t);
+                g1_add(q, q, t);
+            }
         }
     }
-    g1p f = p;
+    g1p f;
+    orig.get(f);
     fe beta2;
     fe_set(beta2, FP_BETA2);
     fe_mul(f.x, f.x, beta2);
@@ -148,6 +164,11 @@ FK20_HD bool g1_in_subgroup(const g1p &p) {
     fe zc;
     fe_canonical(zc, q.z);
     return fe_is_zero_canonical(zc);
+}
+
+FK20_HD bool g1_in_subgroup(const g1p &p) {
+    RegSlot orig, base;
+    return g1_in_subgroup(p, orig, base);
 }
 
 }  // namespace fk20
