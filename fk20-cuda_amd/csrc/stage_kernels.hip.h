@@ -16,13 +16,25 @@ namespace fk20 {
 // (u = L-1), DIF s = u - (L-1).  Butterfly b of a stage reads and writes two positions
 // (in place), given by stage_positions().
 
+// x <- w^(+-e) x, the sign by direction (index[dir * 512 + e] is the program of w^e for TW_FWD,
+// of w^-e for TW_INV): w^+-128 by tw_mul_z3, w^256 = -1 by a negation, any other exponent by
+// its program.  The exponent is wave-uniform, so is the branch.
+template <int W>
+__device__ __forceinline__ void twiddle(g1p &x, const uint32_t *ops, const uint32_t *index, unsigned dir,
+                                        unsigned e) {
+    const unsigned E = (dir == TW_FWD ? e : 512u - e) & 511u;  // x <- w^E x
+    if (E == 128 || E == 384) tw_mul_z3(x, E == 128);
+    else if (E == 256) fe_neg<4>(x.y, x.y);  // stored points have y < 4p
+    else if (E) tw_mul<W>(x, ops + index[dir * 512 + e]);
+}
+
 // IFFT decimation in time (bit-reversed input): x = D[p1] w^-idx; D[p0], D[p1] = y + x, y - x.
 template <int W>
 __device__ __forceinline__ void dit_butterfly(uint32_t *D, unsigned rows, unsigned row, const Butterfly &f,
                                               const uint32_t *ops, const uint32_t *index) {
     g1p x, y, a;
     pt_load(x, D + ((size_t)f.p1 * rows + row) * 36);
-    if (f.idx) tw_mul<W>(x, ops + index[TW_INV * 512 + f.idx]);
+    if (f.idx) twiddle<W>(x, ops, index, TW_INV, f.idx);
     pt_load(y, D + ((size_t)f.p0 * rows + row) * 36);
     g1_add(a, y, x);
     pt_store(D + ((size_t)f.p0 * rows + row) * 36, a);  // stored first: not live in the next add
@@ -38,11 +50,11 @@ __device__ __forceinline__ void middle_butterfly(uint32_t *D, unsigned rows, uns
                                                  const uint32_t *ops, const uint32_t *index) {
     g1p o, e;
     pt_load(o, D + ((size_t)f.p1 * rows + row) * 36);
-    if (f.idx) tw_mul<W>(o, ops + index[TW_INV * 512 + f.idx]);
+    if (f.idx) twiddle<W>(o, ops, index, TW_INV, f.idx);
     pt_load(e, D + ((size_t)f.p0 * rows + row) * 36);
     g1_add(e, e, o);
     pt_store(D + ((size_t)f.p0 * rows + row) * 36, e);
-    if (f.idx) tw_mul<W>(e, ops + index[TW_FWD * 512 + f.idx]);
+    if (f.idx) twiddle<W>(e, ops, index, TW_FWD, f.idx);
     pt_store(D + ((size_t)f.p1 * rows + row) * 36, e);
 }
 
@@ -57,7 +69,7 @@ __device__ __forceinline__ void dif_butterfly(uint32_t *D, unsigned rows, unsign
     pt_store(D + ((size_t)f.p0 * rows + row) * 36, a);  // stored first: not live across tw_mul
     g1_neg(v, v);
     g1_add(v, u, v);
-    if (f.idx) tw_mul<W>(v, ops + index[TW_FWD * 512 + f.idx]);
+    if (f.idx) twiddle<W>(v, ops, index, TW_FWD, f.idx);
     pt_store(D + ((size_t)f.p1 * rows + row) * 36, v);
 }
 
@@ -114,7 +126,7 @@ __global__ void __launch_bounds__(64, OCC) k_twiddle_multi(Launch L, const uint3
     uint32_t *p = it.D + ((size_t)(item & 0xffffu) * it.rows + row) * 36;
     g1p x;
     pt_load(x, p);
-    tw_mul<W>(x, ops + index[dir * 512 + (item >> 16)]);
+    twiddle<W>(x, ops, index, dir, item >> 16);
     pt_store(p, x);
 }
 

@@ -169,6 +169,14 @@ void host_tw_mul_w(uint64_t *r, const uint64_t *p, const uint32_t *ops, int w) {
     stp(r, a);
 }
 
+// The short form of the w^+-128 twiddles (ladder.hip.h): w128 = 1 for w^128, 0 for w^-128.
+void host_tw_mul_z3(uint64_t *r, const uint64_t *p, int w128) {
+    g1p a;
+    ldp(a, p);
+    tw_mul_z3(a, w128 != 0);
+    stp(r, a);
+}
+
 // The kernels' membership test of the order-r subgroup (ladder.hip.h).
 int host_g1_in_subgroup(const uint64_t *p) {
     g1p a;

@@ -299,3 +299,27 @@ def test_subgroup_membership(L):
         assert L.host_g1_in_subgroup(p64(np.ascontiguousarray(p))) == int(want)
     # the random curve points really are outside G1 and their [h] multiples inside
     assert not any(O.g1p_is_inf(O.g1p_op("mul", q, O.fr(R))) for q in curve)
+
+
+def test_w128_short_form(L):
+    """[w^+-128]P by 63 doublings of P and phi^2 (w^128 = |z|^3 = -|z| lambda^2 on G1) equals
+    the oracle's multiple, on random non-normalised points, on G, -G, small multiples and on
+    infinity in (0 : y : 0) form."""
+    rng = np.random.default_rng(71)
+    w = pow(5, (R - 1) // 512, R)
+    s = S.random_fr(24, 72)
+    zs = np.array([O.fp(int.from_bytes(rng.bytes(48), "little") % (P - 1) + 1) for _ in s])
+    pts = list(O.gen_mul_projective(s, zs))
+    G = O.g1p_gen()
+    pts += [G, O.g1p_op("mul", G, O.fr(R - 1)), O.g1p_op("mul", G, O.fr(3)), O.g1p_op("mul", G, O.fr(2))]
+    for w128, e in ((1, 128), (0, 384)):
+        k = pow(w, e, R)
+        for p in pts:
+            r = np.zeros(18, dtype=np.uint64)
+            L.host_tw_mul_z3(p64(r), p64(np.ascontiguousarray(p)), w128)
+            assert O.g1p_eq(r, O.g1p_op("mul", p, O.fr(k)))
+        for y in (1, 7, P - 1):
+            inf = np.concatenate([O.fp(0), O.fp(y), O.fp(0)])
+            r = np.zeros(18, dtype=np.uint64)
+            L.host_tw_mul_z3(p64(r), p64(inf), w128)
+            assert O.g1p_is_inf(r)
