@@ -28,13 +28,21 @@ __device__ __forceinline__ void twiddle(g1p &x, const uint32_t *ops, const uint3
     else if (E) tw_mul<W>(x, ops + index[dir * 512 + e]);
 }
 
+// The butterflies' twiddle: in the radix-2^2 schedule always w^+-128 (idx 128), else any.
+template <int W, bool R22>
+__device__ __forceinline__ void bf_twiddle(g1p &x, const uint32_t *ops, const uint32_t *index, unsigned dir,
+                                           unsigned idx) {
+    if constexpr (R22) tw_mul_z3(x, dir == TW_FWD);
+    else twiddle<W>(x, ops, index, dir, idx);
+}
+
 // IFFT decimation in time (bit-reversed input): x = D[p1] w^-idx; D[p0], D[p1] = y + x, y - x.
-template <int W>
+template <int W, bool R22>
 __device__ __forceinline__ void dit_butterfly(uint32_t *D, unsigned rows, unsigned row, const Butterfly &f,
                                               const uint32_t *ops, const uint32_t *index) {
     g1p x, y, a;
     pt_load(x, D + ((size_t)f.p1 * rows + row) * 36);
-    if (f.idx) twiddle<W>(x, ops, index, TW_INV, f.idx);
+    if (f.idx) bf_twiddle<W, R22>(x, ops, index, TW_INV, f.idx);
     pt_load(y, D + ((size_t)f.p0 * rows + row) * 36);
     g1_add(a, y, x);
     pt_store(D + ((size_t)f.p0 * rows + row) * 36, a);  // stored first: not live in the next add
@@ -45,21 +53,21 @@ __device__ __forceinline__ void dit_butterfly(uint32_t *D, unsigned rows, unsign
 
 // Last IFFT stage, upper half dropped (h = first half), then the first FFT DIF stage whose
 // upper input half is zero:  x = E_j + w^-idx O_j;  D[j] = x;  D[j + n2/2] = w^idx x.
-template <int W>
+template <int W, bool R22>
 __device__ __forceinline__ void middle_butterfly(uint32_t *D, unsigned rows, unsigned row, const Butterfly &f,
                                                  const uint32_t *ops, const uint32_t *index) {
     g1p o, e;
     pt_load(o, D + ((size_t)f.p1 * rows + row) * 36);
-    if (f.idx) twiddle<W>(o, ops, index, TW_INV, f.idx);
+    if (f.idx) bf_twiddle<W, R22>(o, ops, index, TW_INV, f.idx);
     pt_load(e, D + ((size_t)f.p0 * rows + row) * 36);
     g1_add(e, e, o);
     pt_store(D + ((size_t)f.p0 * rows + row) * 36, e);
-    if (f.idx) twiddle<W>(e, ops, index, TW_FWD, f.idx);
+    if (f.idx) bf_twiddle<W, R22>(e, ops, index, TW_FWD, f.idx);
     pt_store(D + ((size_t)f.p1 * rows + row) * 36, e);
 }
 
 // FFT decimation in frequency: D[p0], D[p1] = u + v, (u - v) w^idx.
-template <int W>
+template <int W, bool R22>
 __device__ __forceinline__ void dif_butterfly(uint32_t *D, unsigned rows, unsigned row, const Butterfly &f,
                                               const uint32_t *ops, const uint32_t *index) {
     g1p u, v, a;
@@ -69,7 +77,7 @@ __device__ __forceinline__ void dif_butterfly(uint32_t *D, unsigned rows, unsign
     pt_store(D + ((size_t)f.p0 * rows + row) * 36, a);  // stored first: not live across tw_mul
     g1_neg(v, v);
     g1_add(v, u, v);
-    if (f.idx) twiddle<W>(v, ops, index, TW_FWD, f.idx);
+    if (f.idx) bf_twiddle<W, R22>(v, ops, index, TW_FWD, f.idx);
     pt_store(D + ((size_t)f.p1 * rows + row) * 36, v);
 }
 
@@ -83,9 +91,9 @@ __device__ __forceinline__ void run_butterfly(const Item &it, unsigned u, unsign
     const unsigned rows = it.rows, rg = (rows + 63) / 64;
     const unsigned row = min((local % rg) * 64 + threadIdx.x, rows - 1);
     const Butterfly f = stage_positions<false>(u, it.logn2, local / rg);
-    if (u + 1 < it.logn2) dit_butterfly<W>(it.D, rows, row, f, ops, index);
-    else if (u + 1 == it.logn2) middle_butterfly<W>(it.D, rows, row, f, ops, index);
-    else dif_butterfly<W>(it.D, rows, row, f, ops, index);
+    if (u + 1 < it.logn2) dit_butterfly<W, false>(it.D, rows, row, f, ops, index);
+    else if (u + 1 == it.logn2) middle_butterfly<W, false>(it.D, rows, row, f, ops, index);
+    else dif_butterfly<W, false>(it.D, rows, row, f, ops, index);
 }
 
 // One launch per stage: it.stage = s of this launch's kind (DIT/DIF), 0 for the middle.
@@ -104,7 +112,7 @@ __device__ __forceinline__ void run_butterfly(const Item &it, unsigned u, unsign
         const unsigned rg = (it.rows + 63) / 64;                                                 \
         const unsigned row = min((local % rg) * 64 + threadIdx.x, it.rows - 1);                  \
         const Butterfly f = stage_positions<R22>((USTAGE), it.logn2, local / rg);                \
-        BODY<W>(it.D, it.rows, row, f, ops, index);                                              \
+        BODY<W, R22>(it.D, it.rows, row, f, ops, index);                                         \
     }
 FK20_STAGE_KERNEL(k_dit_multi, dit_butterfly, (unsigned)it.stage)
 FK20_STAGE_KERNEL(k_middle_multi, middle_butterfly, it.logn2 - 1)
