@@ -916,6 +916,8 @@ def main():
         forms["F_1row_latency_ms_p50"] = round(statistics.median(f1), 3)
     # the F output of the last timed step (the graph replays rewrote it)
     gpu_out = b.h_fft.download((b.rows, b.n2, 18))
+    # the uneven and serving legs load their own three setups: free this one's tables first
+    prover.unload_setup(sid)
     uneven, uneven_groups, serve_sids = None, None, {}
     if args.uneven_rows:
         log("timing uneven batch: %d rows each of N=1024/2048/4096 in one pass" % args.uneven_rows)

@@ -384,7 +384,7 @@ extern "C" {
 
 size_t fk20_setup_table_bytes(unsigned N, unsigned window) {
     if (!valid_N(N) || !valid_win(window)) return 0;
-    return (size_t)16 * (N / 8) * nwin_of(window) * ((size_t)1 << (window - 1)) * 96;
+    return (size_t)16 * (N / 8) * nwin_of(window) * ((size_t)1 << (window - 1)) * TABLE_ENTRY_WORDS * 4;
 }
 
 // Per item: tcT (16 n2 Fr per row), D (n2 points per row), the chain flags (n2 words per
@@ -546,7 +546,7 @@ static hipError_t build_tables(Setup &st, const fk20_g1p_t *points, unsigned nba
     uint32_t *xs = nullptr, *zt = nullptr;
     // bases per build launch: one thread per (base, window), batch-inversion scratch bounded
     // by 4 GiB (every base of an N = 4096 setup in one launch up to w = 13)
-    const size_t per_base = (size_t)st.nwin * H * 96;
+    const size_t per_base = (size_t)st.nwin * H * TABLE_ENTRY_WORDS * 4;
     const size_t zt_per_base = table_build_scratch_bytes(1, st.win, st.nwin);
     unsigned chunk = (unsigned)std::max<size_t>(1, ((size_t)4 << 30) / zt_per_base);
     if (chunk > nbases) chunk = nbases;
@@ -647,7 +647,7 @@ static hipError_t order_after(fk20_ctx *c, hipStream_t s) {
 
 size_t fk20_commit_setup_table_bytes(unsigned N, unsigned window) {
     if (!valid_N(N) || !valid_win(window)) return 0;
-    return (size_t)N * nwin_of(window) * ((size_t)1 << (window - 1)) * 96;
+    return (size_t)N * nwin_of(window) * ((size_t)1 << (window - 1)) * TABLE_ENTRY_WORDS * 4;
 }
 
 size_t fk20_commit_workspace_bytes(unsigned rows, unsigned N) {

@@ -57,6 +57,9 @@ hipError_t launch_scale_bases(uint32_t *xs, uint32_t *inf, const uint32_t *xext,
 // cleared first); groups = ceil(count / per_group).
 hipError_t launch_subgroup_groups(uint32_t *flags, const uint32_t *pts, size_t count, size_t per_group,
                                   hipStream_t s);
+// Words per window-table entry: the affine (x, y), 24 words, padded to 32 so that an entry is
+// one 128-byte line of the caches (a 96-byte entry straddles two lines in half the cases).
+static constexpr unsigned TABLE_ENTRY_WORDS = 32;
 // Window tables of bases [base0, base0 + nb): zt is workspace of table_build_scratch_bytes(nb,
 // win_bits, nwin) bytes.
 static constexpr unsigned TABLE_BLK = 256;  // entries per batch inversion

@@ -212,11 +212,11 @@ __global__ void __launch_bounds__(64, 3) k_table_build(uint32_t *table, uint32_t
     if (t >= nthreads) return;
     const unsigned b = base0 + t / nwin, win = t % nwin;
     const unsigned H = 1u << (win_bits - 1), blk = min(H, TABLE_BLK);
-    uint32_t *tab = table + ((size_t)b * nwin + win) * H * G1A_WORDS;
+    uint32_t *tab = table + ((size_t)b * nwin + win) * H * TABLE_ENTRY_WORDS;
     if (inf[b]) {
         for (unsigned e = 0; e < H; e++) {
             uint32_t zero[24] = {0};
-            st_words(tab + (size_t)e * G1A_WORDS, zero, 6);
+            st_words(tab + (size_t)e * TABLE_ENTRY_WORDS, zero, 6);
         }
         return;
     }
@@ -244,8 +244,8 @@ __global__ void __launch_bounds__(64, 3) k_table_build(uint32_t *table, uint32_t
             const unsigned e = e0 + i;
             if (e) g1_add_aff(E, E, bx, by);
             uint32_t *z = zt + ((size_t)i * nthreads + t) * 24;
-            fe_store(tab + (size_t)e * G1A_WORDS, E.x);
-            fe_store(tab + (size_t)e * G1A_WORDS + 12, E.y);
+            fe_store(tab + (size_t)e * TABLE_ENTRY_WORDS, E.x);
+            fe_store(tab + (size_t)e * TABLE_ENTRY_WORDS + 12, E.y);
             fe_store(z, E.z);
             fe_store(z + 12, prefix);  // prefix of z_e0 .. z_{e-1}
             fe_mul(prefix, prefix, E.z);
@@ -263,14 +263,14 @@ __global__ void __launch_bounds__(64, 3) k_table_build(uint32_t *table, uint32_t
             fe_load(pe, z + 12);
             fe_mul(zinv, inv, pe);
             fe_mul(inv, inv, ze);
-            fe_load(x, tab + (size_t)e * G1A_WORDS);
-            fe_load(y, tab + (size_t)e * G1A_WORDS + 12);
+            fe_load(x, tab + (size_t)e * TABLE_ENTRY_WORDS);
+            fe_load(y, tab + (size_t)e * TABLE_ENTRY_WORDS + 12);
             fe_mul(x, x, zinv);
             fe_mul(y, y, zinv);
             fe_canonical(x, x);
             fe_canonical(y, y);
-            fe_store(tab + (size_t)e * G1A_WORDS, x);
-            fe_store(tab + (size_t)e * G1A_WORDS + 12, y);
+            fe_store(tab + (size_t)e * TABLE_ENTRY_WORDS, x);
+            fe_store(tab + (size_t)e * TABLE_ENTRY_WORDS + 12, y);
         }
     }
 }
@@ -355,7 +355,7 @@ __device__ __forceinline__ void window_term(g1p &acc, uint32_t *k, const uint32_
         k[7] >>= WIN;
         if (d != 0) {
             const unsigned e = (unsigned)(d < 0 ? -d : d) - 1;
-            const uint32_t *ent = tb + ((size_t)win * H + e) * G1A_WORDS;
+            const uint32_t *ent = tb + ((size_t)win * H + e) * TABLE_ENTRY_WORDS;
             fe x, y;
             fe_load(x, ent);
             fe_load(y, ent + 12);
@@ -382,7 +382,7 @@ __global__ void __launch_bounds__(64) k_msm_multi(Launch L) {
         if (it.inf[b]) continue;
         uint32_t k[8];
         ld_words(k, it.tcT + (((size_t)j * 16 + i) * rows + row) * FR_WORDS, 2);
-        window_term(acc, k, it.table + (size_t)b * NWIN * H * G1A_WORDS, WIN);
+        window_term(acc, k, it.table + (size_t)b * NWIN * H * TABLE_ENTRY_WORDS, WIN);
     }
     pt_store(it.D + ((size_t)bitrev(j, it.logn2) * rows + row) * 36, acc);
 }
@@ -407,7 +407,7 @@ __global__ void __launch_bounds__(64) k_msm_multi_small(Launch L) {
     if (!it.inf[b]) {
         uint32_t k[8];
         ld_words(k, it.tcT + (((size_t)j * 16 + i) * rows + row) * FR_WORDS, 2);
-        window_term(acc, k, it.table + (size_t)b * NWIN * H * G1A_WORDS, WIN);
+        window_term(acc, k, it.table + (size_t)b * NWIN * H * TABLE_ENTRY_WORDS, WIN);
     }
     pt_store(lds + t * 36, acc);
 #pragma unroll 1
@@ -449,7 +449,7 @@ __global__ void __launch_bounds__(64) k_commit_partial(const uint32_t *poly, con
         fs k;
         ld_words(k.l, poly + ((size_t)row * N + i) * FR_WORDS, 2);
         fs_reduce_std(k);
-        window_term(acc, k.l, table + (size_t)i * NWIN * H * G1A_WORDS, win_bits);
+        window_term(acc, k.l, table + (size_t)i * NWIN * H * TABLE_ENTRY_WORDS, win_bits);
     }
     pt_store(partial + ((size_t)c * rows + row) * 36, acc);
 }

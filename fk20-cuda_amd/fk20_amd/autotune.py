@@ -15,7 +15,7 @@ N and batch size (tuned.json next to the library); bench.py and callers read the
 The reference fixes the same kinds of choice at compile time (CUDAGRAPH, fk20benchmark.cu:5;
 FFT_Sharedmem, fk20benchmark.cu:302; SHAREDMEMROOTS, fk20_hext_fft2h_fft.cu:4).
 
-    python3 -m fk20_amd.autotune [--rows 512,64,1] [--sizes 1024,2048,4096] [--max-table-gb 72]
+    python3 -m fk20_amd.autotune [--rows 512,64,1] [--sizes 1024,2048,4096] [--max-table-gb 96]
                                  [--mixed-rows 512]
 """
 import argparse
@@ -113,7 +113,7 @@ def _time(prover, sid, poly_dev, out_dev, rows, reps):
     return statistics.median(ms)
 
 
-def tune(sizes=(1024, 2048, 4096), rows_list=(512, 64, 1), max_table_gb=72.0, reps=3, seed=11, xext_for=None,
+def tune(sizes=(1024, 2048, 4096), rows_list=(512, 64, 1), max_table_gb=96.0, reps=3, seed=11, xext_for=None,
          windows=CANDIDATE_WINDOWS, schedules=None):
     """xext_for(N) -> DeviceBuffer of xext_fft[16][N/8] (a synthetic setup by default)."""
     if xext_for is None:
@@ -218,7 +218,7 @@ def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--rows", default="512,64,1", help="batch sizes to tune the schedule for")
     ap.add_argument("--sizes", default="1024,2048,4096")
-    ap.add_argument("--max-table-gb", type=float, default=72.0)
+    ap.add_argument("--max-table-gb", type=float, default=96.0)
     ap.add_argument("--reps", type=int, default=3)
     ap.add_argument("--mixed-rows", type=int, default=512,
                     help="rows per size of the uneven batch whose schedule is tuned last (0: skip)")

@@ -204,7 +204,8 @@ int fk20_ctx_get_output(fk20_ctx_t *ctx, int *format);
 /* Bytes per output point of a format (144 or 48; 0 for an unknown format). */
 size_t fk20_output_point_bytes(int format);
 /* Device bytes of one setup's window tables for polynomial length N and window width w
- * (8 to 13 bits): 16 * (N/8) * ceil(256/w) * 2^(w-1) * 96, e.g. 35 GiB at N=4096, w=12. */
+ * (8 to 13 bits): 16 * (N/8) * ceil(256/w) * 2^(w-1) * 128 (an affine entry padded to one
+ * 128-byte cache line), e.g. 80 GiB at N=4096, w=13. */
 size_t fk20_setup_table_bytes(unsigned N, unsigned window);
 /* Builds the resident tables of xext_fft[16][N/8] (device memory, standard form);
  * synchronises `stream`.  The returned id names the setup in fk20_prove_rows.  FK20_EINVAL

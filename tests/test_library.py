@@ -77,9 +77,10 @@ def test_twiddle_programs_selfcheck():
 
 def test_setup_table_bytes():
     from fk20_amd import prover
-    assert prover.setup_table_bytes(4096, 8) == 16 * 512 * 32 * 128 * 96
-    assert prover.setup_table_bytes(1024, 10) == 16 * 128 * 26 * 512 * 96
-    assert prover.setup_table_bytes(4096, 13) == 16 * 512 * 20 * 4096 * 96
+    # entries are affine (x, y), 96 bytes, padded to 128
+    assert prover.setup_table_bytes(4096, 8) == 16 * 512 * 32 * 128 * 128
+    assert prover.setup_table_bytes(1024, 10) == 16 * 128 * 26 * 512 * 128
+    assert prover.setup_table_bytes(4096, 13) == 16 * 512 * 20 * 4096 * 128
     assert prover.setup_table_bytes(4096, 7) == 0
     assert prover.setup_table_bytes(4096, 14) == 0
 
@@ -133,7 +134,7 @@ def test_output_format_sizes_and_argument_checks():
 
 def test_commit_sizes_and_argument_checks():
     lib = L.lib()
-    assert lib.fk20_commit_setup_table_bytes(4096, 8) == 4096 * 32 * 128 * 96
+    assert lib.fk20_commit_setup_table_bytes(4096, 8) == 4096 * 32 * 128 * 128
     assert lib.fk20_commit_setup_table_bytes(4000, 8) == 0
     assert lib.fk20_commit_workspace_bytes(512, 4096) == (256 + 16) * 512 * 144
     assert lib.fk20_commit_setup_load(None, None, 4096, 8, None, None) == -1
