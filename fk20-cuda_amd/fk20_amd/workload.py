@@ -100,6 +100,8 @@ def launch_mul32(N, kind, rows):
 #     doubling with update 7, seven co-Z additions 6 each, rescaling 34, table Z 1), one
 #     Jacobian doubling (7) per doubling op, one mixed addition (11) per digit after the first,
 #     one Fp mul per phi-digit, conversion back 4 and the infinity test 1
+#   twiddle w^+-128: tw_mul_z3 -- conversion in and the infinity test 4, 63 Jacobian doublings
+#     (7) and 5 mixed additions (11) of [|z|] x, conversion out with phi^2 5; w^256: a negation
 #   butterfly: twiddle (if e != 0) + 2 complete additions (12 each)
 #   MSM: one mixed addition (11 Fp mul) per non-zero signed window digit, 16 terms per column
 
@@ -141,7 +143,17 @@ def twiddle_program(e_idx, inverse=False, wnaf=WNAF_W):
     return _programs[key]
 
 
+Z3_LADDER = 4 + 63 * J_DBL + 5 * J_MADD + 5
+
+
 def twiddle_fp_muls(e_idx, inverse=False, wnaf=WNAF_W):
+    """Executed Fp muls of the transform kernels' multiplication by w^(+-e_idx)
+    (stage_kernels.hip.h twiddle): the short forms of w^+-128 and w^256, else the program."""
+    e = (-e_idx if inverse else e_idx) % 512
+    if e in (128, 384):
+        return Z3_LADDER
+    if e in (0, 256):
+        return 0
     dbl, adds, phi, _ = twiddle_program(e_idx, inverse, wnaf)
     return ladder_fixed(wnaf) + dbl * J_DBL + adds * J_MADD + phi
 
