@@ -138,6 +138,8 @@ struct StageKernels {
     void (*middle)(Launch, const uint32_t *, const uint32_t *) = nullptr;
     void (*dif)(Launch, const uint32_t *, const uint32_t *) = nullptr;
     void (*twiddle)(Launch, const uint32_t *, const uint32_t *, unsigned) = nullptr;
+    // radix-2^2 only: the w^128 twiddles of a DIF stage, after its butterflies (k_quarter_multi)
+    void (*quarter)(Launch) = nullptr;
     void (*chain)(Chain, const uint32_t *, const uint32_t *) = nullptr;
     void (*plain_load)(uint32_t *, const uint32_t *, unsigned, unsigned, unsigned, const uint32_t *) = nullptr;
     int wnaf = 0;

@@ -702,6 +702,18 @@ hipError_t launch_batch(const Launch &base, const DevTables *T, const uint32_t *
             total = set_blocks(L, per);
             if (total) hipLaunchKernelGGL(K.dif, dim3(total), dim3(64), 0, s, L, ops, index);
             hook(LK_DIF);
+            // even DIF stages (radix-2^2): their w^128 twiddles, on the second half's outputs
+            for (unsigned k = 0; k < L.nitems; k++) {
+                const Item &it = L.item[k];
+                const bool q = K.r22 && t + 1 < it.logn2 && (t & 1) == 0;
+                L.item[k].stage = q ? (int)t : -1;
+                per[k] = q ? (1u << (it.logn2 - 2)) * ((it.rows + 63) / 64) : 0;
+            }
+            total = set_blocks(L, per);
+            if (total) {
+                hipLaunchKernelGGL(K.quarter, dim3(total), dim3(64), 0, s, L);
+                hook(LK_DIF);
+            }
             // odd DIF stages: their twiddle layer (radix-2^2)
             for (unsigned k = 0; k < L.nitems; k++) {
                 const Item &it = L.item[k];
@@ -773,6 +785,10 @@ hipError_t launch_plain(PlainKind kind, uint32_t *out, const uint32_t *in, const
         per[0] = n / 2 * rg;
         const unsigned total = set_blocks(L, per);
         hipLaunchKernelGGL(kernel, dim3(total), dim3(64), 0, s, L, P.ops, P.index);
+        if (dif && (st & 1) == 0 && (unsigned)st + 1 < logn) {  // its w^128 twiddles
+            per[0] = (n / 4) * rg;
+            hipLaunchKernelGGL(K.quarter, dim3(set_blocks(L, per)), dim3(64), 0, s, L);
+        }
         if (dif) layer((unsigned)st, (unsigned)TW_FWD);
     };
     switch (kind) {

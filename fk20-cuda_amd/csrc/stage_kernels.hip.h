@@ -66,7 +66,10 @@ __device__ __forceinline__ void middle_butterfly(uint32_t *D, unsigned rows, uns
     pt_store(D + ((size_t)f.p1 * rows + row) * 36, e);
 }
 
-// FFT decimation in frequency: D[p0], D[p1] = u + v, (u - v) w^idx.
+// FFT decimation in frequency: D[p0], D[p1] = u + v, (u - v) w^idx.  In the radix-2^2
+// schedule the w^128 of a quarter stage is applied by its own launch after the butterflies
+// (k_quarter_multi): in the butterfly kernel those ladders ran at 0.9 ms per launch at 512
+// rows where the same ladders in a separate launch take 0.5 ms.
 template <int W, bool R22>
 __device__ __forceinline__ void dif_butterfly(uint32_t *D, unsigned rows, unsigned row, const Butterfly &f,
                                               const uint32_t *ops, const uint32_t *index) {
@@ -77,7 +80,7 @@ __device__ __forceinline__ void dif_butterfly(uint32_t *D, unsigned rows, unsign
     pt_store(D + ((size_t)f.p0 * rows + row) * 36, a);  // stored first: not live across tw_mul
     g1_neg(v, v);
     g1_add(v, u, v);
-    if (f.idx) bf_twiddle<W, R22>(v, ops, index, TW_FWD, f.idx);
+    if (!R22 && f.idx) bf_twiddle<W, R22>(v, ops, index, TW_FWD, f.idx);
     pt_store(D + ((size_t)f.p1 * rows + row) * 36, v);
 }
 
@@ -135,6 +138,24 @@ __global__ void __launch_bounds__(64, OCC) k_twiddle_multi(Launch L, const uint3
     g1p x;
     pt_load(x, p);
     twiddle<W>(x, ops, index, dir, item >> 16);
+    pt_store(p, x);
+}
+
+// The w^128 twiddles of radix-2^2 DIF quarter stage it.stage (>= 1; stage 0 is inside the
+// middle), after its butterflies: one wave per (twiddled butterfly, 64-row group), the
+// butterflies of the launch's second half, whose difference output D[p1] takes w^128.
+template <int W, bool R22, int OCC>
+__global__ void __launch_bounds__(64, OCC) k_quarter_multi(Launch L) {
+    unsigned local;
+    const Item &it = L.item[find_item(L, blockIdx.x, local)];
+    const unsigned rg = (it.rows + 63) / 64;
+    const unsigned row = min((local % rg) * 64 + threadIdx.x, it.rows - 1);
+    const unsigned b = (1u << (it.logn2 - 2)) + local / rg;
+    const Butterfly f = stage_positions<true>(it.logn2 - 1 + (unsigned)it.stage, it.logn2, b);
+    uint32_t *p = it.D + ((size_t)f.p1 * it.rows + row) * 36;
+    g1p x;
+    pt_load(x, p);
+    tw_mul_z3(x, true);
     pt_store(p, x);
 }
 
@@ -228,6 +249,7 @@ constexpr StageKernels make_stage_kernels() {
     K.dif = k_dif_multi<W, R22, OCC>;
     K.twiddle = k_twiddle_multi<W, R22, OCC>;
     if constexpr (!R22) K.chain = k_chain<W, R22, OCC>;
+    if constexpr (R22) K.quarter = k_quarter_multi<W, R22, OCC>;
     K.plain_load = k_plain_load<W, R22, OCC>;
     K.wnaf = W;
     K.r22 = R22;
