@@ -31,7 +31,8 @@ struct Butterfly {
 // ones first, the twiddled ones in the second half of the launch (b -> block and offset with
 // the offset's top bit taken from b's top bit).  Measured on MI355X at 512 rows: the IFFT's
 // w^-128 launches 1.1 -> 0.67 ms (blocks in launch order gave some SIMDs two ladders and
-// others none); the FFT's w^128 launches, which follow a twiddle layer, stayed at 1.1 ms.
+// others none).  The FFT's w^128 ladders run in a launch of their own after the butterflies
+// (k_quarter_multi, over the second half of this numbering).
 FK20_HD unsigned quarter_order(unsigned b, unsigned logn2, unsigned lh, unsigned &j) {
     const unsigned c = b >> (logn2 - 2), r = b & ((1u << (logn2 - 2)) - 1);
     j = (r & ((1u << (lh - 1)) - 1)) + (c << (lh - 1));

@@ -37,7 +37,7 @@ def mixed_rows(n, seed):
     return x
 
 
-@pytest.mark.parametrize("n", [128, 512])
+@pytest.mark.parametrize("n", [128, 256, 512])
 def test_g1_transforms_outside_g1(n):
     x = mixed_rows(n, 500 + n)
     fwd = ops.g1p_fft(x)
