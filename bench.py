@@ -542,10 +542,13 @@ def bench_commit(args):
     stream.destroy()
 
 
-def serve_levels(prover, sids, rates, duration, budget_bytes, slots, max_batch_rows, seed, stop_past_saturation=True):
+def serve_levels(prover, sids, rates, duration, budget_bytes, slots, max_batch_rows, seed, stop_past_saturation=True,
+                 warmup_s=1.0):
     """Poisson arrivals of one-row blobs, N = 1024/2048/4096 weighted 1:2:5, into the serving
     loop (fk20_server_*) over `prover`'s resident setups sids[N], at each offered load in
-    rows/s for `duration` seconds; latency = completion - submission of each request."""
+    rows/s for `duration` seconds; latency = completion - submission of each request.  An
+    untimed level of `warmup_s` seconds at the highest load runs first (the first level of a
+    process ran 20-50 % slower at p50 than the same level repeated)."""
     import threading
 
     from fk20_amd.lib import PinnedArray
@@ -559,10 +562,11 @@ def serve_levels(prover, sids, rates, duration, budget_bytes, slots, max_batch_r
         outs[N] = PinnedArray((pool, N // 8, 18))
     sizes = np.array([1024] * 1 + [2048] * 2 + [4096] * 5)
     levels = []
-    for rate in rates:
+    plan = ([(max(rates), warmup_s, False)] if warmup_s > 0 else []) + [(r, duration, True) for r in rates]
+    for rate, secs, timed in plan:
         srv = Server(prover, budget_bytes=budget_bytes, slots=slots, max_batch_rows=max_batch_rows)
-        rng = np.random.default_rng(seed + int(rate))
-        n = max(64, int(rate * duration))
+        rng = np.random.default_rng(seed + int(rate) + (0 if timed else 7))
+        n = max(64, int(rate * secs))
         gaps = rng.exponential(1.0 / rate, size=n)
         which = sizes[rng.integers(0, len(sizes), size=n)]
         ids = [None] * n
@@ -595,6 +599,10 @@ def serve_levels(prover, sids, rates, duration, budget_bytes, slots, max_batch_r
         st = srv.stats()
         srv.close()
         lat_ms = np.array(lat) * 1e3
+        if not timed:
+            log("serve warm-up: %.0f rows/s offered for %.1f s, p50 %.2f ms (not reported)"
+                % (rate, secs, float(np.percentile(lat_ms, 50))))
+            continue
         levels.append({
             "offered_rows_per_s": rate,
             "sustained_rows_per_s": round(n / (t_end - t0), 1),
@@ -644,7 +652,7 @@ def bench_serve(args):
         "data": "synthetic one-row blobs, N=1024/2048/4096 weighted 1:2:5, Poisson arrivals",
         "config": {"workload": "fk20_serving", "budget_bytes": args.budget_gb << 30, "slots": args.slots,
                    "max_batch_rows": args.max_batch_rows, "msm_window_bits": args.window,
-                   "duration_s_per_level": args.duration},
+                   "duration_s_per_level": args.duration, "warmup_s": 1.0},
         "levels": levels,
     }
     print(json.dumps(out), flush=True)
@@ -942,7 +950,7 @@ def main():
         levels = serve_levels(prover, serve_sids, rates, args.duration, args.budget_gb << 30, args.slots,
                               args.max_batch_rows, args.seed, stop_past_saturation=False)
         serving = {"budget_bytes": args.budget_gb << 30, "slots": args.slots,
-                   "max_batch_rows": args.max_batch_rows, "duration_s_per_level": args.duration,
+                   "max_batch_rows": args.max_batch_rows, "duration_s_per_level": args.duration, "warmup_s": 1.0,
                    "blobs": "one row each, N=1024/2048/4096 weighted 1:2:5, Poisson arrivals",
                    "p50_ms": levels[-1]["p50_ms"], "p99_ms": levels[-1]["p99_ms"],
                    "offered_rows_per_s": levels[-1]["offered_rows_per_s"],
