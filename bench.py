@@ -690,7 +690,7 @@ def cpu_baseline(b, threads, runs, gpu_out, uneven_groups):
     O.lib()
     avail, total, model = host_cpus()
     if threads <= 0:
-        threads = max(1, min(16, avail))  # the per-GPU CPU share of a GPU box
+        threads = max(1, min(16, avail))  # at most 16 host threads per GPU
     runs = max(1, min(runs, b.rows // threads))
     times, cpu_rows = [], []
     for k in range(runs):
