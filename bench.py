@@ -518,7 +518,7 @@ def bench_commit(args):
     exact = None
     if not args.no_cpu:
         sys.path.insert(0, os.path.join(ROOT, "oracle"))
-        import oracle as O  # the checker
+        import oracle as O  # the parity reference
         exact = all(np.array_equal(O.to_affine(got[r]), O.to_affine(O.commit_closed_form(poly_host[r], tau)))
                     for r in range(min(rows, 8)))
     p50 = statistics.median(ms)

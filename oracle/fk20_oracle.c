@@ -1,7 +1,7 @@
 /*
  * fk20_oracle.c -- CPU restatement of the FK20 multi-proof path.
  *
- * THIS IS TEST INFRASTRUCTURE (the oracle), NOT PRODUCT CODE.  It is the checker
+ * THIS IS TEST INFRASTRUCTURE (the oracle), NOT PRODUCT CODE.  It is the parity reference
  * for the HIP library and the timed CPU baseline; the product never links it.
  *
  * Algorithm sources restated here (reference = lCardosoSantos/FK20-CUDA):

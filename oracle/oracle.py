@@ -1,7 +1,7 @@
 """ctypes binding of the CPU oracle (libfk20_oracle.so).
 
 TEST INFRASTRUCTURE ONLY: imported by tests/, by the smoke check in __graft_entry__.py
-and by the cpu_baseline leg of bench.py, always as the checker or the timed CPU baseline.
+and by the cpu_baseline leg of bench.py, always as the parity reference or the timed CPU baseline.
 
 Arrays use the reference ABI (FK20-CUDA fr.cuh:24, g1.cuh:20-30) as numpy uint64:
   fr  -> shape (..., 4)    g1p -> shape (..., 18)    g1a -> shape (..., 12)
