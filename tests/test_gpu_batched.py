@@ -75,6 +75,33 @@ def test_batched_edge_inputs(prover):
     prover.unload_setup(sid)
 
 
+def test_batched_msm_exceptional_sums(prover):
+    """Column sums of the batched MSM that add a point to itself or to its negative: every
+    setup row holds the same point Q_j (row 1 holds -Q_j for odd j), and polynomials whose
+    top 16 coefficients are c (the rest 0) have Toeplitz coefficients c at position 0 only,
+    so every MSM scalar is c and the second term meets the first one's sum.  Against the
+    plain path and the oracle; 20 rows, so the row-parallel MSM kernel runs."""
+    N, rows = 1024, 20
+    n2 = N // 8
+    q = ops.g1p_op("mul", np.tile(O.g1p_gen(), (n2, 1)), k=S.random_fr(n2, 570))
+    xext = np.ascontiguousarray(np.repeat(q[None], 16, axis=0))
+    for j in range(1, n2, 2):
+        y = O.limbs_to_int(xext[1, j, 6:12])
+        xext[1, j, 6:12] = O.fp((O.P_MOD - y) % O.P_MOD)
+    poly = S.random_polys(rows, N, 571)
+    for r, c in ((0, 1), (2, 2), (5, 4095)):
+        poly[r] = 0
+        poly[r, N - 16:] = O.fr(c)
+    poly[3] = 0
+    sid = prover.load_setup(xext, N, 8)
+    got = prover.prove(sid, poly)
+    assert np.array_equal(aff(got), aff(ops.poly2h_fft(poly, xext)))
+    for r in (0, 2, 5):
+        assert np.array_equal(aff(got[r]), aff(O.poly2h_fft(poly[r:r + 1], xext))), r
+    assert all(O.g1p_is_inf(p) for p in got[3])
+    prover.unload_setup(sid)
+
+
 def test_graph_replay_tracks_input_contents():
     from fk20_amd.lib import DeviceBuffer, Stream
     N, rows = 1024, 64
